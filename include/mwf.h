@@ -1,0 +1,120 @@
+/*
+ * mwf.h -- C-ABI of libmwf.so, the MI355X timestepper for Model Waleffe Flow.
+ *
+ * The reference exposes its per-step work as Fortran module procedures acting
+ * on module-global state (module velocity, program/velocity.f90:15-17,21-162,
+ * and module transform, program/transform.fftw5.f90:33-145), selected at build
+ * time by the Makefile's TRANSFORM/MODES variables (Makefile:6-7,68-80).  This
+ * library replaces that work on one GPU.  The Fortran velocity variant in
+ * mwf_amd/fortran/ binds these entry points with ISO_C_BINDING.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a negative MWF_E* code;
+ *    mwf_last_error() returns a message for the calling thread's last error.
+ *  - State arrays use the reference layout of type(mpt) (program/variables.f90
+ *    :12-15): separate Re and Im planes, column-major (K, 0:M-1, 0:NN-1),
+ *    element (k, m, n) at k + K*(m + M*n), K = 2*K0-1, M = 2*MM-1.
+ *  - phys arrays use type(phys) (:8-10): (KK, 0:Q-1, 0:P-1), element (c, z, x)
+ *    at c + KK*(z + Q*x), KK = 3*K0-1, P = 3*MM, Q = 3*NN.
+ *  - Calls on one handle come from one host thread.  GPU work is queued on the
+ *    handle's own HIP stream; entry points that return host data synchronise it.
+ */
+#ifndef MWF_H
+#define MWF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MWF_OK 0
+#define MWF_E_ARG -1      /* invalid argument or unsupported size */
+#define MWF_E_HIP -2      /* HIP runtime error (message has the HIP error) */
+#define MWF_E_NOMEM -3    /* device or host allocation failed */
+#define MWF_E_STATE -4    /* call out of order (e.g. step before set_state) */
+
+#define MWF_FP64 0        /* every kernel in FP64 */
+#define MWF_MIXED 1       /* FFTs and products FP32; state, AB2, update, energy FP64 */
+
+/* Model parameters: the compile-time constants of program/parameters.f90
+ * (:7-46) as run-time values.  theta is in radians (d_theta, :61). */
+typedef struct mwf_params {
+    int32_t k0;         /* i_K0: 4 (modes.M4.f90) or 6 (modes.M6.f90) */
+    int32_t mm;         /* i_MM, power of two, 16..1024 */
+    int32_t nn;         /* i_NN, power of two, 16..512 */
+    int32_t dragall;    /* s_dragall */
+    int32_t precision;  /* MWF_FP64 or MWF_MIXED */
+    int32_t reserved;   /* must be 0 */
+    double alpha;       /* d_alpha = 2 pi / Lx */
+    double gamma;       /* d_gamma = 2 pi / Lz */
+    double re;          /* d_Re */
+    double dt;          /* d_dt */
+    double drag;        /* d_drag */
+    double vdrag;       /* d_vdrag */
+    double hypo;        /* d_HYPO */
+    double theta;       /* d_theta */
+} mwf_params;
+
+typedef struct mwf_handle mwf_handle;
+
+/* vel_precompute + var_precompute + tra_precompute (program/velocity.f90:43-50,
+ * program/variables.f90:488-533, program/transform.fftw5.f90:33-59): device
+ * buffers, twiddle tables and launch plans.  The state starts at zero. */
+int mwf_create(const mwf_params *p, mwf_handle **out);
+int mwf_destroy(mwf_handle *h);
+
+/* Upload vel_c (what io_load_mpt fills, program/io.f90:153,166-204); marks the
+ * next step as the first (nst = .true., program/velocity.f90:48). */
+int mwf_set_state(mwf_handle *h, const double *re, const double *im);
+/* Download vel_c (what io_save_state reads, program/io.f90:257-290). */
+int mwf_get_state(mwf_handle *h, double *re, double *im);
+
+/* nsteps x vel_TS (program/velocity.f90:21-41): nonlinear term, AB2 (forward
+ * Euler on the first step) and the diagonal implicit update. */
+int mwf_step(mwf_handle *h, int32_t nsteps);
+
+/* vel_mpt2phys(vel_c) + vel_energy (program/io.f90:98-99,
+ * program/velocity.f90:84-90,144-162): perturbation energy of the current
+ * state, reduced on the device (spectral Parseval sum, compensated). */
+int mwf_energy(mwf_handle *h, double *e);
+
+/* vel_nonlinear (program/velocity.f90:92-113) of the current state, without
+ * stepping: the mpt-form nonlinear term vel_nl. */
+int mwf_nonlinear(mwf_handle *h, double *nlre, double *nlim);
+
+/* vel_mpt2phys (program/velocity.f90:84-90): mpt -> phys for any host mpt. */
+int mwf_mpt2phys(mwf_handle *h, const double *re, const double *im, double *phys);
+
+/* Wait for all queued work of the handle. */
+int mwf_sync(mwf_handle *h);
+
+/* Message of the calling thread's last error ("" if none). */
+const char *mwf_last_error(void);
+
+/* Seeded Gaussian-spectrum initial condition, host only.  The reference ships
+ * no IC generator (README.md:7-13 names a randIC utility absent from the
+ * tree; program/variables.f90:233-286 draw unseeded uniforms); this one
+ * defines the synthetic inputs:  SplitMix64(seed) + Box-Muller; per mode
+ * (n outer, m, then spec slot) u, v, w coefficients
+ *   scl * exp(-(kx^2 + kz^2)/k0^2) * (N(0,1) + i N(0,1))/sqrt(2),
+ * kx = alpha*m (signed), kz = gamma*n; projected per wavenumber onto the
+ * divergence-free toroidal/poloidal form; zero deviation at (0,0) (laminar
+ * mean); n = 0 column made Hermitian.  Writes an mpt in the layout above. */
+int mwf_ic_gaussian(const mwf_params *p, uint64_t seed, double scl, double k0,
+                    double *re, double *im);
+
+/* Per-kernel timing with HIP events on the handle's stream.  kernel ids:
+ * 0 = x-inverse, 1 = z fused, 2 = x-forward + update (+ next x-inverse),
+ * 3 = energy.  total_ms is the summed event time since profiling was enabled. */
+int mwf_profile_enable(mwf_handle *h, int32_t on);
+int mwf_profile_read(mwf_handle *h, int32_t kernel, double *total_ms, int64_t *count);
+
+/* Algorithmic HBM bytes one launch of a kernel id moves (inputs read once,
+ * outputs written once), for roofline accounting. */
+int mwf_kernel_bytes(mwf_handle *h, int32_t kernel, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

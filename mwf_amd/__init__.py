@@ -1,0 +1,192 @@
+"""MI355X-native Model Waleffe Flow timestepper (Python host mirror).
+
+The compute runs in ``libmwf.so`` (hand-written HIP for gfx950, C-ABI in
+``include/mwf.h``).  This module binds it with ctypes and mirrors the
+reference's ``velocity`` module interface (program/velocity.f90):
+
+=====================  ================================================
+reference              here
+=====================  ================================================
+``vel_precompute``     ``Velocity(params)``
+``vel_c`` (mpt)        ``Velocity.vel_c`` (get) / ``set_state``
+``vel_TS``             ``Velocity.vel_TS(nsteps=1)``
+``vel_nonlinear``      ``Velocity.vel_nonlinear()`` (returns vel_nl)
+``vel_mpt2phys``       ``Velocity.vel_mpt2phys(re, im)``
+``vel_energy``         ``Velocity.vel_energy()`` (of the current vel_c)
+=====================  ================================================
+
+mpt arrays are numpy float64 of shape ``(NN, M, K)`` -- the memory layout of
+the reference's ``mpt%Re(K, 0:M-1, 0:NN-1)``; phys arrays ``(P, Q, KK)``.
+There is no CPU fallback: if the library is missing or the GPU call fails,
+an ``MWFError`` is raised.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libmwf.so")
+
+MWF_FP64 = 0
+MWF_MIXED = 1
+
+KID_XINV, KID_Z, KID_XFWD, KID_ENERGY = 0, 1, 2, 3
+KERNEL_NAMES = {KID_XINV: "k_xinv", KID_Z: "k_z", KID_XFWD: "k_xfwd", KID_ENERGY: "energy"}
+
+
+class MWFError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    """mwf_params of include/mwf.h (program/parameters.f90 as run-time values)."""
+    _fields_ = [("k0", ctypes.c_int32), ("mm", ctypes.c_int32), ("nn", ctypes.c_int32),
+                ("dragall", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("alpha", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("re", ctypes.c_double), ("dt", ctypes.c_double),
+                ("drag", ctypes.c_double), ("vdrag", ctypes.c_double), ("hypo", ctypes.c_double),
+                ("theta", ctypes.c_double)]
+
+
+def make_params(k0, mm, nn, lx, lz, re, dt, drag=1e-2, vdrag=0.0, hypo=0.0, thdeg=0.0,
+                dragall=True, precision=MWF_FP64):
+    """Parameters with program/parameters.f90 defaults (drag 1e-2, dragall, no
+    hypo-viscosity, theta = 0)."""
+    return Params(int(k0), int(mm), int(nn), int(bool(dragall)), int(precision), 0,
+                  2.0 * math.pi / lx, 2.0 * math.pi / lz, float(re), float(dt), float(drag),
+                  float(vdrag), float(hypo), thdeg / 180.0 * math.pi)
+
+
+_LIB = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    """Load libmwf.so (built by mwf_amd.build) -- raises if it is missing."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise MWFError(f"{LIBPATH} is not built; run `python -m mwf_amd.build`")
+        L = ctypes.CDLL(LIBPATH)
+        vp = ctypes.c_void_p
+        L.mwf_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
+        L.mwf_destroy.argtypes = [vp]
+        L.mwf_set_state.argtypes = [vp, _dp, _dp]
+        L.mwf_get_state.argtypes = [vp, _dp, _dp]
+        L.mwf_step.argtypes = [vp, ctypes.c_int32]
+        L.mwf_energy.argtypes = [vp, _dp]
+        L.mwf_nonlinear.argtypes = [vp, _dp, _dp]
+        L.mwf_mpt2phys.argtypes = [vp, _dp, _dp, _dp]
+        L.mwf_sync.argtypes = [vp]
+        L.mwf_last_error.restype = ctypes.c_char_p
+        L.mwf_ic_gaussian.argtypes = [ctypes.POINTER(Params), ctypes.c_uint64, ctypes.c_double,
+                                      ctypes.c_double, _dp, _dp]
+        L.mwf_profile_enable.argtypes = [vp, ctypes.c_int32]
+        L.mwf_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64)]
+        L.mwf_kernel_bytes.argtypes = [vp, ctypes.c_int32, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise MWFError(f"libmwf error {rc}: {lib().mwf_last_error().decode()}")
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def dims(p):
+    k0, mm, nn = p.k0, p.mm, p.nn
+    return dict(K=2 * k0 - 1, KK=3 * k0 - 1, M=2 * mm - 1, P=3 * mm, Q=3 * nn)
+
+
+def ic_gaussian(p, seed, scl, k0):
+    """Seeded Gaussian-spectrum divergence-free initial condition (mwf_ic_gaussian)."""
+    d = dims(p)
+    re = np.empty((p.nn, d["M"], d["K"]))
+    im = np.empty_like(re)
+    _check(lib().mwf_ic_gaussian(ctypes.byref(p), ctypes.c_uint64(seed), float(scl), float(k0),
+                                 _p(re), _p(im)))
+    return re, im
+
+
+class Velocity:
+    """One model instance on the GPU: the reference's module velocity."""
+
+    def __init__(self, params):
+        self.params = params
+        d = dims(params)
+        self.K, self.KK, self.M, self.P, self.Q = d["K"], d["KK"], d["M"], d["P"], d["Q"]
+        self.NN = params.nn
+        h = ctypes.c_void_p()
+        _check(lib().mwf_create(ctypes.byref(params), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mwf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def mpt_shape(self):
+        return (self.NN, self.M, self.K)
+
+    def set_state(self, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64).reshape(self.mpt_shape)
+        im = np.ascontiguousarray(im, dtype=np.float64).reshape(self.mpt_shape)
+        _check(lib().mwf_set_state(self._h, _p(re), _p(im)))
+
+    def get_state(self):
+        re, im = np.empty(self.mpt_shape), np.empty(self.mpt_shape)
+        _check(lib().mwf_get_state(self._h, _p(re), _p(im)))
+        return re, im
+
+    vel_c = property(get_state)
+
+    def vel_TS(self, nsteps=1):
+        _check(lib().mwf_step(self._h, int(nsteps)))
+
+    def vel_energy(self):
+        e = ctypes.c_double()
+        _check(lib().mwf_energy(self._h, ctypes.byref(e)))
+        return e.value
+
+    def vel_nonlinear(self):
+        re, im = np.empty(self.mpt_shape), np.empty(self.mpt_shape)
+        _check(lib().mwf_nonlinear(self._h, _p(re), _p(im)))
+        return re, im
+
+    def vel_mpt2phys(self, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64).reshape(self.mpt_shape)
+        im = np.ascontiguousarray(im, dtype=np.float64).reshape(self.mpt_shape)
+        ph = np.empty((self.P, self.Q, self.KK))
+        _check(lib().mwf_mpt2phys(self._h, _p(re), _p(im), _p(ph)))
+        return ph
+
+    def sync(self):
+        _check(lib().mwf_sync(self._h))
+
+    # per-kernel timing
+    def profile(self, on=True):
+        _check(lib().mwf_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self, kid):
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        _check(lib().mwf_profile_read(self._h, kid, ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    def kernel_bytes(self, kid):
+        b = ctypes.c_double()
+        _check(lib().mwf_kernel_bytes(self._h, kid, ctypes.byref(b)))
+        return b.value

@@ -1,0 +1,56 @@
+// common.hpp -- complex arithmetic, geometry and compile-time helpers for the
+// gfx950 kernels of the Model Waleffe Flow timestepper.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+#include <utility>
+
+template <typename R> struct cplx_of;
+template <> struct cplx_of<double> { using type = double2; };
+template <> struct cplx_of<float> { using type = float2; };
+template <typename R> using cplx_t = typename cplx_of<R>::type;
+
+template <typename CT> __device__ __forceinline__ CT cadd(CT a, CT b) { return CT{a.x + b.x, a.y + b.y}; }
+template <typename CT> __device__ __forceinline__ CT csub(CT a, CT b) { return CT{a.x - b.x, a.y - b.y}; }
+template <typename CT> __device__ __forceinline__ CT cmul(CT a, CT b)
+{
+    return CT{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename CT> __device__ __forceinline__ CT cconj(CT a) { return CT{a.x, -a.y}; }
+template <typename CT> __device__ __forceinline__ CT cscale(CT a, decltype(a.x) s) { return CT{a.x * s, a.y * s}; }
+// multiply by SIGN * i
+template <int SIGN, typename CT> __device__ __forceinline__ CT cmuli(CT a)
+{
+    return SIGN > 0 ? CT{-a.y, a.x} : CT{a.y, -a.x};
+}
+
+template <typename CT> __device__ __forceinline__ CT ld_tw(const CT* __restrict__ tw, int e)
+{
+    return tw[e];
+}
+
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int B, int E, typename F> __device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(std::forward<F>(f));
+    }
+}
+
+// Geometry and physical parameters of one model instance.
+struct Geo {
+    int K0, K, KK;     // y-modes, mpt slots (2K0-1), spec slots (3K0-1)
+    int MM, M, ldM;    // x: modes |m| < MM, M = 2MM-1 stored, ldM = row pitch
+    int NN, P, Q;      // z modes, padded grid P = 3MM (x), Q = 3NN (z)
+    int NTx;           // z-modes per x-kernel block
+    int XB;            // x-rows per z-kernel block
+    double alpha, gamma, re, dt, drag, vdrag, hypo, cth, sth, beta;
+    double scale;      // 1/(P*Q), transform.fftw5.f90:242
+    int dragall;
+};
+
+// wall-normal derivative table ad_k1, ad_k2 (program/variables.f90:528-531)
+__device__ __forceinline__ double ak1_of(int k, double beta) { return ((k & 1) ? -1.0 : 1.0) * k * beta; }
+__device__ __forceinline__ double ak2_of(int k, double beta) { return -k * k * beta * beta; }

@@ -1,0 +1,637 @@
+// kernels.hpp -- the per-step kernels of the Model Waleffe Flow timestepper.
+//
+// One step (program/velocity.f90:21-41) is three launches:
+//   k_xinv : state c  -> U   mpt2spec + laminar + d/dx, padded inverse x-DFT
+//   k_z    : U        -> NL  z c2r of u, du/dx, du/dz; udotgradu; z r2c
+//   k_xfwd : NL, c    -> c   forward x-DFT, spec2mpt, AB2 + implicit update
+//
+// Device layouts (complex values interleaved):
+//   c, onl   double2 [NN][K][ldM]      m in the reference order (m<0 at M+m)
+//   U        cplx<R> [P][2*KK][NN]     x-physical, z-spectral; lines 0..KK-1
+//                                      are u (spec slot order), KK..2KK-1 du/dx
+//   NL       cplx<R> [P][KK][NN]       x-physical, z-spectral nonlinear term
+//
+// Padded transforms.  The x transform of length P = 3*MM over modes
+// |m| <= MM-1 is split with x = 3a + b into three length-MM transforms of the
+// folded spectra Y_b[r] = X[r] w^{rb} + X[r-MM] w^{(r-MM)b}, w = exp(2 pi i/P)
+// (inverse), and the forward one into three length-MM transforms of the
+// decimated rows followed by X[m] = sum_b w^{-mb} F_b[m mod MM].  The z
+// transforms (length Q = 3*NN, real data) fold the same way; two real rows
+// share one complex transform.
+#pragma once
+#include "common.hpp"
+#include "fft.hpp"
+#include "galerkin_gen.hpp"
+
+constexpr int XT = 512;     // threads per block of the x kernels
+constexpr int XTILE = 2048; // NTx * MM: x-kernel tile (complex values per line)
+constexpr int ZT = 512;     // threads per block of the z kernels (= XB * NN)
+constexpr int ZPB = 2;      // component pairs per z batch
+
+// x wavenumber tables of program/variables.f90:516-523 for storage index mi.
+__device__ __forceinline__ void xwave(const Geo& g, int mi, double& am1, double& am2)
+{
+    const int ma = mi < g.MM ? mi : g.M - mi; // |m|
+    const double a = g.alpha * ma;
+    am1 = mi < g.MM ? a : -a;
+    am2 = -(a * g.alpha * ma);
+}
+
+__device__ __forceinline__ void zwave(const Geo& g, int n, double& an1, double& an2)
+{
+    const double c = g.gamma * n;
+    an1 = c;
+    an2 = -(c * g.gamma * n);
+}
+
+// var_mpt2spec (program/variables.f90:390-434) for y-mode k at one (m, n):
+// from psi_k (slot k) and phi_k (slot K0-1+k, k >= 1) to u_k, v_k, w_k.
+// At (m,n) = (0,0) the mean-mode rows apply (:418-433) and, when lam is set,
+// the laminar profile of vel_addlam (program/velocity.f90:76-82).
+__device__ __forceinline__ void mpt2spec_k(const Geo& g, int k, int mi, int n, double2 psi, double2 phi,
+                                           bool lam, double2& u, double2& v, double2& w)
+{
+#pragma clang fp contract(off)
+    double am1, am2, an1, an2;
+    xwave(g, mi, am1, am2);
+    zwave(g, n, an1, an2);
+    const double ak1 = ak1_of(k, g.beta);
+    if (k == 0) phi = double2{0.0, 0.0};
+    u = double2{-(-(psi.y * an1) + phi.y * ak1 * am1), (-(psi.x * an1) + phi.x * ak1 * am1)};
+    v = double2{-(phi.x * (am2 + an2)), -(phi.y * (am2 + an2))};
+    w = double2{-(psi.y * am1 + phi.y * ak1 * an1), (psi.x * am1 + phi.x * ak1 * an1)};
+    if (mi == 0 && n == 0) {
+        if (k == 0) {
+            u = double2{0.0, 0.0};
+            w = double2{0.0, 0.0};
+        } else {
+            u = psi;
+            w = phi;
+            if (lam && k == 1) {
+                u.x = u.x + g.cth;
+                w.x = w.x + g.sth;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x inverse: block (n-tile, y-mode k).  Writes U for the k-group lines and,
+// when epart is set, the block's share of the perturbation energy (Parseval
+// of the spec fields without the laminar profile, program/velocity.f90:115-162).
+// ---------------------------------------------------------------------------
+template <typename R, int LOGMM>
+__global__ __launch_bounds__(XT) void k_xinv(Geo g, const double2* __restrict__ c, cplx_t<R>* __restrict__ U,
+                                             double* __restrict__ epart, const cplx_t<R>* __restrict__ twMM,
+                                             const double2* __restrict__ twP, int addlam)
+{
+    using CT = cplx_t<R>;
+    constexpr int MM = 1 << LOGMM;
+    constexpr int IPT = XTILE / XT; // items per thread
+    extern __shared__ unsigned char smem_raw[];
+    CT* buf = reinterpret_cast<CT*>(smem_raw);
+    __shared__ double red[XT];
+
+    const int NT = g.NTx;
+    const int n0 = blockIdx.x * NT;
+    const int k = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int nitems = NT * MM;
+    const int nlines3 = k == 0 ? 2 : 3;
+    const int slot_psi = k, slot_phi = g.K0 - 1 + k;
+    const int spec_slot[3] = {k, g.K0 - 1 + k, 2 * g.K0 - 1 + k};
+
+    // spec values at m = r (h = 0) and m = r - MM (h = 1): u, v, w
+    double2 sp[IPT][2][3];
+    double esum = 0.0;
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int it = tid + q * XT;
+        if (it >= nitems) continue;
+        const int t = it >> LOGMM, r = it & (MM - 1);
+        const int n = n0 + t;
+        const double2* cn = c + (size_t)n * g.K * g.ldM;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int mi = h == 0 ? r : MM - 1 + r;
+            if (h == 1 && r == 0) {
+                sp[q][h][0] = sp[q][h][1] = sp[q][h][2] = double2{0.0, 0.0};
+                continue;
+            }
+            const double2 psi = cn[(size_t)slot_psi * g.ldM + mi];
+            const double2 phi = k > 0 ? cn[(size_t)slot_phi * g.ldM + mi] : double2{0.0, 0.0};
+            double2 u, v, w;
+            mpt2spec_k(g, k, mi, n, psi, phi, false, u, v, w);
+            if (epart) {
+                // y weights of epos: 1 for the k = 0 mean of u and w, 1/2 otherwise
+                const double yw = k == 0 ? 1.0 : 0.5;
+                if (n > 0) {
+                    esum += 2.0 * yw * (u.x * u.x + u.y * u.y + w.x * w.x + w.y * w.y);
+                    if (k > 0) esum += 2.0 * yw * (v.x * v.x + v.y * v.y);
+                } else if (h == 0) {
+                    // n = 0: the c2r keeps the Hermitian part of the column
+                    if (r == 0) {
+                        esum += yw * (u.x * u.x + w.x * w.x);
+                    } else {
+                        const int mneg = g.M - r;
+                        const double2 psn = cn[(size_t)slot_psi * g.ldM + mneg];
+                        const double2 phn = k > 0 ? cn[(size_t)slot_phi * g.ldM + mneg] : double2{0.0, 0.0};
+                        double2 un, vn, wn;
+                        mpt2spec_k(g, k, mneg, n, psn, phn, false, un, vn, wn);
+                        const double hu_r = 0.5 * (u.x + un.x), hu_i = 0.5 * (u.y - un.y);
+                        const double hv_r = 0.5 * (v.x + vn.x), hv_i = 0.5 * (v.y - vn.y);
+                        const double hw_r = 0.5 * (w.x + wn.x), hw_i = 0.5 * (w.y - wn.y);
+                        esum += 2.0 * yw * (hu_r * hu_r + hu_i * hu_i + hw_r * hw_r + hw_i * hw_i);
+                        if (k > 0) esum += 2.0 * yw * (hv_r * hv_r + hv_i * hv_i);
+                    }
+                }
+            }
+            if (addlam) mpt2spec_k(g, k, mi, n, psi, phi, true, u, v, w);
+            sp[q][h][0] = u;
+            sp[q][h][1] = v;
+            sp[q][h][2] = w;
+        }
+    }
+
+    if (epart) {
+        red[tid] = esum;
+        __syncthreads();
+        for (int s = XT / 2; s > 0; s >>= 1) {
+            if (tid < s) red[tid] += red[tid + s];
+            __syncthreads();
+        }
+        if (tid == 0) epart[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+    }
+    if (!U) return;
+
+#pragma unroll
+    for (int li = 0; li < 2 * 3; li++) {
+        const int comp = li % 3, deriv = li / 3;
+        if (nlines3 == 2 && comp == 1) continue;
+        const int line = spec_slot[comp] + deriv * g.KK;
+        // fold into Y_b[r], b = 0..2
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const int it = tid + q * XT;
+            if (it >= nitems) continue;
+            const int t = it >> LOGMM, r = it & (MM - 1);
+            double2 X1 = sp[q][0][comp], X2 = sp[q][1][comp];
+            if (deriv) { // i alpha m (program/variables.f90:606-607)
+                double am1, am2;
+                xwave(g, r, am1, am2);
+                X1 = double2{-(X1.y * am1), X1.x * am1};
+                xwave(g, MM - 1 + r, am1, am2);
+                X2 = double2{-(X2.y * am1), X2.x * am1};
+            }
+            CT* bt = buf + (size_t)t * 3 * MM;
+            bt[r] = CT{(R)(X1.x + X2.x), (R)(X1.y + X2.y)};
+#pragma unroll
+            for (int b = 1; b < 3; b++) {
+                const double2 w1 = twP[r * b];
+                const double2 w2 = twP[(MM - r) * b]; // conj: exp(2 pi i (r-MM) b / P)
+                const double y_r = (X1.x * w1.x - X1.y * w1.y) + (X2.x * w2.x + X2.y * w2.y);
+                const double y_i = (X1.x * w1.y + X1.y * w1.x) + (X2.y * w2.x - X2.x * w2.y);
+                bt[b * MM + r] = CT{(R)y_r, (R)y_i};
+            }
+        }
+        __syncthreads();
+        lds_fft<CT, LOGMM, +1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
+        // write U[x][line][n0 + t]
+#pragma unroll
+        for (int q = 0; q < 3 * IPT; q++) {
+            const int it = tid + q * XT;
+            if (it >= 3 * nitems) continue;
+            const int x = it / NT, t = it - x * NT;
+            const int a = x / 3, b = x - 3 * a;
+            U[((size_t)x * 2 * g.KK + line) * g.NN + n0 + t] = buf[((size_t)t * 3 + b) * MM + a];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z fused: block of XB = ZT/NN x-rows; thread (row xl, a) owns the physical
+// points z = 3a + b, b = 0..2 of its row.
+// ---------------------------------------------------------------------------
+template <typename R, int K0, int LOGNN>
+__device__ __forceinline__ cplx_t<R> zload(const Geo& g, const cplx_t<R>* __restrict__ Urow, int gi, int n)
+{
+    using CT = cplx_t<R>;
+    constexpr int KK = 3 * K0 - 1;
+    if (gi >= 3 * KK) return CT{(R)0, (R)0};
+    const int fam = gi / KK, s = gi - fam * KK;
+    constexpr int NN = 1 << LOGNN;
+    if (fam == 1) return Urow[(KK + s) * NN + n];
+    CT v = Urow[s * NN + n];
+    if (fam == 2) { // i gamma n (program/variables.f90:608-609)
+        const R an1 = (R)(g.gamma * n);
+        v = CT{-(v.y * an1), v.x * an1};
+    }
+    return v;
+}
+
+template <typename R, int K0, int LOGNN>
+__global__ __launch_bounds__(ZT) void k_z(Geo g, const cplx_t<R>* __restrict__ U, cplx_t<R>* __restrict__ NL,
+                                          const cplx_t<R>* __restrict__ twNN, const cplx_t<R>* __restrict__ twQ)
+{
+    using CT = cplx_t<R>;
+    using G = Gal<K0>;
+    constexpr int KK = 3 * K0 - 1;
+    constexpr int NN = 1 << LOGNN;
+    constexpr int XB = ZT / NN;
+    constexpr int NC = 3 * KK;                             // u, du/dx, du/dz
+    constexpr int NBATCH = (NC + 2 * ZPB - 1) / (2 * ZPB); // inverse batches
+    constexpr int NPA = (KK + 1) / 2;                      // output pairs
+    constexpr int NROUND = (NPA + ZPB - 1) / ZPB;          // forward rounds
+    constexpr int SEQ = XB * ZPB * 3;
+    extern __shared__ unsigned char smem_raw[];
+    CT* buf = reinterpret_cast<CT*>(smem_raw);
+
+    const int tid = threadIdx.x;
+    const int xl = tid >> LOGNN, r = tid & (NN - 1);
+    const int x = blockIdx.x * XB + xl;
+    const bool valid = x < g.P;
+    const CT* Urow = U + (size_t)(valid ? x : 0) * 2 * KK * NN;
+    const int Q = 3 * NN;
+
+    R vb[3][KK];
+    R an[3][KK];
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int s = 0; s < KK; s++) an[b][s] = (R)0;
+
+    static_for<0, NBATCH>([&](auto bi) {
+        constexpr int B = decltype(bi)::value;
+        // fold two real rows per complex sequence
+        static_for<0, ZPB>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+            constexpr int g1 = B * 2 * ZPB + 2 * p, g2 = g1 + 1;
+            CT Z[3];
+            if (valid && g1 < NC) {
+                const CT a1 = zload<R, K0, LOGNN>(g, Urow, g1, r);
+                const CT a2 = zload<R, K0, LOGNN>(g, Urow, g2, r);
+                if (r == 0) {
+                    Z[0] = Z[1] = Z[2] = CT{a1.x, a2.x};
+                } else {
+                    const CT b1 = zload<R, K0, LOGNN>(g, Urow, g1, NN - r);
+                    const CT b2 = zload<R, K0, LOGNN>(g, Urow, g2, NN - r);
+                    Z[0] = CT{(a1.x + b1.x) - (a2.y - b2.y), (a1.y - b1.y) + (a2.x + b2.x)};
+#pragma unroll
+                    for (int b = 1; b < 3; b++) {
+                        const CT w1 = twQ[r * b];
+                        const CT w2 = twQ[(NN - r) * b]; // conj -> exp(-2 pi i (NN-r) b / Q)
+                        // Y = a w1 + conj(b) conj(w2) = a w1 + conj(b w2)
+                        const CT bw1 = cmul(b1, w2), bw2 = cmul(b2, w2);
+                        const CT y1 = cadd(cmul(a1, w1), cconj(bw1));
+                        const CT y2 = cadd(cmul(a2, w1), cconj(bw2));
+                        Z[b] = CT{y1.x - y2.y, y1.y + y2.x};
+                    }
+                }
+            } else {
+                Z[0] = Z[1] = Z[2] = CT{(R)0, (R)0};
+            }
+#pragma unroll
+            for (int b = 0; b < 3; b++) buf[((xl * ZPB + p) * 3 + b) * NN + r] = Z[b];
+        });
+        __syncthreads();
+        lds_fft<CT, LOGNN, +1, ZT, SEQ>(buf, SEQ, twNN);
+        // consume: thread (xl, a = r)
+        static_for<0, ZPB>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const CT zv = buf[((xl * ZPB + p) * 3 + b) * NN + r];
+                static_for<0, 2>([&](auto hi) {
+                    constexpr int gi = B * 2 * ZPB + 2 * p + decltype(hi)::value;
+                    if constexpr (gi < NC) {
+                        const R val = decltype(hi)::value == 0 ? zv.x : zv.y;
+                        constexpr int fam = gi / KK, s = gi % KK;
+                        if constexpr (fam == 0) vb[b][s] = val;
+                        else if constexpr (fam == 1) G::template dx<s>(vb[b], val, an[b]);
+                        else G::template dz<s>(vb[b], val, an[b]);
+                    }
+                });
+            }
+        });
+        __syncthreads();
+    });
+#pragma unroll
+    for (int b = 0; b < 3; b++) G::yterms(vb[b], an[b], (R)g.beta);
+
+    // forward z transform of the KK products, two per complex sequence
+    const R scale = (R)g.scale;
+    static_for<0, NROUND>([&](auto ri) {
+        constexpr int RD = decltype(ri)::value;
+        static_for<0, ZPB>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+            constexpr int c1 = 2 * (RD * ZPB + p), c2 = c1 + 1;
+            if constexpr (c1 < KK) {
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    R v2 = (R)0;
+                    if constexpr (c2 < KK) v2 = an[b][c2];
+                    buf[((xl * ZPB + p) * 3 + b) * NN + r] = CT{an[b][c1], v2};
+                }
+            }
+        });
+        __syncthreads();
+        lds_fft<CT, LOGNN, -1, ZT, SEQ>(buf, SEQ, twNN);
+        static_for<0, ZPB>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+            constexpr int c1 = 2 * (RD * ZPB + p), c2 = c1 + 1;
+            if constexpr (c1 < KK) {
+                CT X1{(R)0, (R)0}, X2{(R)0, (R)0};
+                const int rm = (NN - r) & (NN - 1);
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    const CT zn = buf[((xl * ZPB + p) * 3 + b) * NN + r];
+                    const CT zm = buf[((xl * ZPB + p) * 3 + b) * NN + rm];
+                    const CT F1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
+                    const CT F2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
+                    if (b == 0) {
+                        X1 = F1;
+                        X2 = F2;
+                    } else {
+                        const CT w = cconj(twQ[r * b]);
+                        X1 = cadd(X1, cmul(F1, w));
+                        X2 = cadd(X2, cmul(F2, w));
+                    }
+                }
+                if (valid) {
+                    CT* out = NL + (size_t)x * KK * NN;
+                    out[c1 * NN + r] = cscale(X1, scale);
+                    if constexpr (c2 < KK) out[c2 * NN + r] = cscale(X2, scale);
+                }
+            }
+        });
+        __syncthreads();
+    });
+}
+
+// z inverse only, to the reference phys layout (vel_mpt2phys).
+template <typename R, int K0, int LOGNN>
+__global__ __launch_bounds__(ZT) void k_zphys(Geo g, const cplx_t<R>* __restrict__ U, double* __restrict__ phys,
+                                              const cplx_t<R>* __restrict__ twNN, const cplx_t<R>* __restrict__ twQ)
+{
+    using CT = cplx_t<R>;
+    constexpr int KK = 3 * K0 - 1;
+    constexpr int NN = 1 << LOGNN;
+    constexpr int XB = ZT / NN;
+    constexpr int NBATCH = (KK + 2 * ZPB - 1) / (2 * ZPB);
+    constexpr int SEQ = XB * ZPB * 3;
+    extern __shared__ unsigned char smem_raw[];
+    CT* buf = reinterpret_cast<CT*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int xl = tid >> LOGNN, r = tid & (NN - 1);
+    const int x = blockIdx.x * XB + xl;
+    const bool valid = x < g.P;
+    const CT* Urow = U + (size_t)(valid ? x : 0) * 2 * KK * NN;
+    const int Q = 3 * NN;
+    for (int B = 0; B < NBATCH; B++) {
+        for (int p = 0; p < ZPB; p++) {
+            const int g1 = B * 2 * ZPB + 2 * p, g2 = g1 + 1;
+            CT Z[3];
+            if (valid && g1 < KK) {
+                const CT a1 = Urow[g1 * NN + r];
+                const CT a2 = g2 < KK ? Urow[g2 * NN + r] : CT{(R)0, (R)0};
+                if (r == 0) {
+                    Z[0] = Z[1] = Z[2] = CT{a1.x, a2.x};
+                } else {
+                    const CT b1 = Urow[g1 * NN + NN - r];
+                    const CT b2 = g2 < KK ? Urow[g2 * NN + NN - r] : CT{(R)0, (R)0};
+                    Z[0] = CT{(a1.x + b1.x) - (a2.y - b2.y), (a1.y - b1.y) + (a2.x + b2.x)};
+                    for (int b = 1; b < 3; b++) {
+                        const CT w1 = twQ[r * b];
+                        const CT w2 = twQ[(NN - r) * b];
+                        const CT y1 = cadd(cmul(a1, w1), cconj(cmul(b1, w2)));
+                        const CT y2 = cadd(cmul(a2, w1), cconj(cmul(b2, w2)));
+                        Z[b] = CT{y1.x - y2.y, y1.y + y2.x};
+                    }
+                }
+            } else {
+                Z[0] = Z[1] = Z[2] = CT{(R)0, (R)0};
+            }
+            for (int b = 0; b < 3; b++) buf[((xl * ZPB + p) * 3 + b) * NN + r] = Z[b];
+        }
+        __syncthreads();
+        lds_fft<CT, LOGNN, +1, ZT, SEQ>(buf, SEQ, twNN);
+        for (int p = 0; p < ZPB; p++) {
+            for (int b = 0; b < 3; b++) {
+                const CT zv = buf[((xl * ZPB + p) * 3 + b) * NN + r];
+                const int g1 = B * 2 * ZPB + 2 * p;
+                const int z = 3 * r + b;
+                if (valid && g1 < KK) phys[((size_t)x * Q + z) * KK + g1] = (double)zv.x;
+                if (valid && g1 + 1 < KK) phys[((size_t)x * Q + z) * KK + g1 + 1] = (double)zv.y;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+
+// var_spec2mpt (program/variables.f90:440-480), split by the order in which
+// the x-forward kernel produces u_k, v_k, w_k.
+__device__ __forceinline__ double2 s2m_vterm(double2 v, double am2, double an2)
+{
+#pragma clang fp contract(off)
+    return double2{v.x * (am2 + an2), v.y * (am2 + an2)};
+}
+
+__device__ __forceinline__ void s2m_finish(int k, int mi, int n, double2 u, double2 w, double2 vh, double am1,
+                                           double an1, double ak1, double2& psi, double2& phi)
+{
+#pragma clang fp contract(off)
+    psi = double2{(w.y * am1 - u.y * an1), -(w.x * am1 - u.x * an1)};
+    if (k > 0) phi = double2{-(u.y * am1 * ak1 + w.y * an1 * ak1) + vh.x, (u.x * am1 * ak1 + w.x * an1 * ak1) + vh.y};
+    if (mi == 0 && n == 0) {
+        if (k == 0) {
+            psi = double2{0.0, 0.0};
+        } else {
+            psi = double2{-u.x, -u.y};
+            phi = double2{-w.x, -w.y};
+        }
+    }
+}
+
+// var_LHSRHS (program/variables.f90:565-597) for psi_k (which = 0) or
+// phi_k (which = 1) at one (m, n).
+__device__ __forceinline__ void lhsrhs_k(const Geo& g, int k, int which, bool mean, double am2, double an2,
+                                         double ak2, double& lhs, double& rhs)
+{
+#pragma clang fp contract(off)
+    const double dt = g.dt;
+    const double lap = (am2 + ak2) + an2;
+    const double hlap = am2 + an2;
+    const double ilap16 = g.hypo / lap;
+    const double tvl = (g.dragall || k == 0) ? 1.0 - dt * ((lap / g.re + ilap16) - g.drag)
+                                             : 1.0 - dt * (lap / g.re + ilap16);
+    if (mean) {
+        if (which == 0 && k == 0) {
+            lhs = 1.0;
+            rhs = 0.0;
+        } else {
+            lhs = tvl;
+            rhs = 1.0;
+        }
+    } else if (which == 0) {
+        lhs = tvl * hlap;
+        rhs = hlap;
+    } else {
+        lhs = g.dragall ? tvl * hlap * lap + dt * (g.vdrag - g.drag) * hlap * hlap : tvl * hlap * lap;
+        rhs = hlap * lap;
+    }
+}
+
+// vel_TS update (program/velocity.f90:28-35) in the reference's order
+__device__ __forceinline__ double2 ab2_update(double2 cv, double2 nl, double2 ov, double lhs, double rhs, double dt)
+{
+#pragma clang fp contract(off)
+    double cr = rhs * cv.x;
+    double ci = rhs * cv.y;
+    cr = cr + dt / 2.0 * (3.0 * nl.x - ov.x);
+    ci = ci + dt / 2.0 * (3.0 * nl.y - ov.y);
+    return double2{cr / lhs, ci / lhs};
+}
+
+// ---------------------------------------------------------------------------
+// x forward + spec2mpt + update: block (n-tile, y-mode k).
+// mode 0: AB2/implicit update of c and onl (vel_TS); nst = first step.
+// mode 1: write the nonlinear term into onl only (vel_nonlinear).
+// ---------------------------------------------------------------------------
+template <typename R, int LOGMM>
+__global__ __launch_bounds__(XT) void k_xfwd(Geo g, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
+                                             double2* __restrict__ onl, const cplx_t<R>* __restrict__ twMM,
+                                             const double2* __restrict__ twP, int mode, int nst)
+{
+    using CT = cplx_t<R>;
+    constexpr int MM = 1 << LOGMM;
+    constexpr int IPT = XTILE / XT;
+    extern __shared__ unsigned char smem_raw[];
+    CT* buf = reinterpret_cast<CT*>(smem_raw);
+
+    const int NT = g.NTx;
+    const int n0 = blockIdx.x * NT;
+    const int k = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int nitems = NT * MM;
+    const double ak1 = ak1_of(k, g.beta);
+
+    // psi[] holds u_k until w_k arrives, phi[] holds v_k (am2+an2) until then
+    double2 psi[IPT][2], phi[IPT][2];
+
+    for (int comp = 0; comp < 3; comp++) {
+        if (k == 0 && comp == 1) continue;
+        const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
+        // load NL rows, decimated by b = x mod 3
+#pragma unroll
+        for (int q = 0; q < 3 * IPT; q++) {
+            const int it = tid + q * XT;
+            if (it >= 3 * nitems) continue;
+            const int x = it / NT, t = it - x * NT;
+            const int a = x / 3, b = x - 3 * a;
+            buf[((size_t)t * 3 + b) * MM + a] = NL[((size_t)x * g.KK + slot) * g.NN + n0 + t];
+        }
+        __syncthreads();
+        lds_fft<CT, LOGMM, -1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const int it = tid + q * XT;
+            if (it >= nitems) continue;
+            const int t = it >> LOGMM, r = it & (MM - 1);
+            const int n = n0 + t;
+            const CT* bt = buf + (size_t)t * 3 * MM;
+            double2 Xp, Xn;
+            {
+                const CT F0 = bt[r];
+                Xp = double2{(double)F0.x, (double)F0.y};
+                Xn = Xp;
+#pragma unroll
+                for (int b = 1; b < 3; b++) {
+                    const CT Fb = bt[b * MM + r];
+                    const double fr = (double)Fb.x, fi = (double)Fb.y;
+                    const double2 w1 = twP[r * b];        // m = r:      times conj(w1)
+                    const double2 w2 = twP[(MM - r) * b]; // m = r - MM: times w2
+                    Xp.x += fr * w1.x + fi * w1.y;
+                    Xp.y += fi * w1.x - fr * w1.y;
+                    Xn.x += fr * w2.x - fi * w2.y;
+                    Xn.y += fi * w2.x + fr * w2.y;
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const double2 X = h == 0 ? Xp : Xn;
+                if (h == 1 && r == 0) continue;
+                const int mi = h == 0 ? r : MM - 1 + r;
+                double am1, am2, an1, an2;
+                xwave(g, mi, am1, am2);
+                zwave(g, n, an1, an2);
+                // var_spec2mpt (program/variables.f90:440-480)
+                if (comp == 0) {
+                    psi[q][h] = X;
+                } else if (comp == 1) {
+                    phi[q][h] = s2m_vterm(X, am2, an2);
+                } else {
+                    const double2 u = psi[q][h], vh = phi[q][h];
+                    s2m_finish(k, mi, n, u, X, vh, am1, an1, ak1, psi[q][h], phi[q][h]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // update (program/velocity.f90:24-40) with var_LHSRHS (program/variables.f90
+    // :565-597) evaluated in place
+    const double dt = g.dt;
+    const double ak2 = ak2_of(k, g.beta);
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int it = tid + q * XT;
+        if (it >= nitems) continue;
+        const int t = it >> LOGMM, r = it & (MM - 1);
+        const int n = n0 + t;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if (h == 1 && r == 0) continue;
+            const int mi = h == 0 ? r : MM - 1 + r;
+            double am1, am2, an1, an2;
+            xwave(g, mi, am1, am2);
+            zwave(g, n, an1, an2);
+            const bool mean = (mi == 0 && n == 0);
+#pragma unroll
+            for (int which = 0; which < 2; which++) {
+                if (which == 1 && k == 0) continue;
+                const int slot = which == 0 ? k : g.K0 - 1 + k;
+                const size_t idx = ((size_t)n * g.K + slot) * g.ldM + mi;
+                const double2 nl = which == 0 ? psi[q][h] : phi[q][h];
+                if (mode == 1) {
+                    onl[idx] = nl;
+                    continue;
+                }
+                double lhs, rhs;
+                lhsrhs_k(g, k, which, mean, am2, an2, ak2, lhs, rhs);
+                const double2 cv = c[idx];
+                const double2 ov = nst ? nl : onl[idx];
+                double2 cn = ab2_update(cv, nl, ov, lhs, rhs, dt);
+                if (mean && which == 0 && k == 0) cn.x = 0.0;
+                c[idx] = cn;
+                onl[idx] = nl;
+            }
+        }
+    }
+}
+
+// Sum of the energy partials in a fixed order with compensation.
+__global__ void k_ereduce(const double* __restrict__ part, int n, double* __restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0, comp = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double y = part[i] - comp;
+            const double t = s + y;
+            comp = (t - s) - y;
+            s = t;
+        }
+        out[0] = s;
+    }
+}
