@@ -1,0 +1,144 @@
+"""Throughput of the MI355X Model Waleffe Flow step on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: the M4 basis, 1024 x 1024 Fourier modes (i_MM = i_NN = 512, a
+1536 x 1536 padded grid), Lx = Lz = 128 pi, Re = 180, dt = 0.02, seeded
+Gaussian-spectrum initial condition (seed 2, rms 0.3, k0 = 8), FP64.  A step
+is one vel_TS (program/velocity.f90:21-41): nonlinear term through the
+3/2-dealiased transforms, AB2 and the implicit update.  The state is resident
+in HBM for the whole timed region.  The same workload is then run in the
+mixed form (FP32 transforms and products, FP64 state/update/energy).
+
+Prints one JSON line.  ``value`` is FP64 steps/s; ``roofline`` is the
+dominant kernel's algorithmic bytes per launch over its mean event-timed
+duration against 8 TB/s; ``cpu_baseline`` times the oracle (the repository's
+C restatement of the reference path with its own FFT, FFTW being absent) on
+a bounded sample of the same workload on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "steps/sec FP64 vs mixed at M4 1024²; ensemble member-steps/sec (1 MI355X)"
+PEAK_HBM_GBS = 8000.0
+WORKLOAD = dict(k0=4, mm=512, nn=512, lx=128 * np.pi, lz=128 * np.pi, re=180.0, dt=0.02)
+IC = dict(seed=2, scl=0.3, k0=8.0)
+
+
+def run_gpu(mw, precision, steps, warmup, ic):
+    p = mw.make_params(precision=precision, **WORKLOAD)
+    v = mw.Velocity(p)
+    v.set_state(*ic)
+    v.vel_TS(warmup)
+    v.sync()
+    v.profile(True)
+    v.sync()
+    t0 = time.perf_counter()
+    v.vel_TS(steps)
+    v.sync()
+    t1 = time.perf_counter()
+    kern = {}
+    for kid, name in mw.KERNEL_NAMES.items():
+        ms, cnt = v.profile_read(kid)
+        if cnt:
+            kern[name] = dict(kid=kid, total_ms=ms, count=cnt, avg_ms=ms / cnt,
+                              bytes=v.kernel_bytes(kid))
+    v.profile(False)
+    re, im = v.get_state()
+    finite = bool(np.isfinite(re).all() and np.isfinite(im).all())
+    e = v.vel_energy()
+    v.close()
+    return dict(seconds=t1 - t0, steps_per_s=steps / (t1 - t0), kernels=kern, finite=finite,
+                energy=e)
+
+
+def cpu_baseline(ic, nsteps):
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    oracle.set_threads(threads)
+    w = WORKLOAD
+    import math
+    o = oracle.Oracle(w["k0"], w["mm"], w["nn"], 2 * math.pi / w["lx"], 2 * math.pi / w["lz"],
+                      w["re"], w["dt"])
+    o.set_state(*ic)
+    o.step(1)  # first step (forward Euler) untimed
+    t0 = time.perf_counter()
+    o.step(nsteps)
+    t1 = time.perf_counter()
+    return dict(value=nsteps / (t1 - t0), unit="steps/s", cores=threads, kind="port",
+                sample=f"{nsteps} steps of the same workload (M4 1024^2 modes, FP64) after one "
+                       f"untimed step; oracle = C restatement of the reference path with its own "
+                       f"FFT (FFTW 3.3.8 unavailable offline), OpenMP over x/z lines")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import mwf_amd as mw
+    p = mw.make_params(**WORKLOAD)
+    ic = mw.ic_gaussian(p, IC["seed"], IC["scl"], IC["k0"])
+
+    fp64 = run_gpu(mw, mw.MWF_FP64, args.steps, args.warmup, ic)
+    mixed = None if args.no_mixed else run_gpu(mw, mw.MWF_MIXED, args.steps, args.warmup, ic)
+
+    kern = fp64["kernels"]
+    dom_name, dom = max(kern.items(), key=lambda kv: kv[1]["total_ms"])
+    achieved = dom["bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
+    step_bytes = sum(k["bytes"] for n, k in kern.items() if n != "energy")
+    line = {
+        "metric": METRIC,
+        "value": fp64["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / fp64["steps_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "M4 basis, 1024x1024 Fourier modes (MM=NN=512, 1536^2 padded grid), "
+                               "Lx=Lz=128pi, Re=180, dt=0.02, seeded Gaussian IC (seed 2, rms 0.3, k0 8)",
+                   "form": "FP64"},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "bytes_per_launch": dom["bytes"], "avg_ms": dom["avg_ms"]},
+        "step_roofline": {"bytes_per_step": step_bytes,
+                          "achieved_GBs": step_bytes * fp64["steps_per_s"] / 1e9,
+                          "frac": step_bytes * fp64["steps_per_s"] / 1e9 / PEAK_HBM_GBS},
+        "kernels_ms": {n: round(k["avg_ms"], 5) for n, k in kern.items()},
+        "finite": fp64["finite"],
+    }
+    if mixed:
+        line["mixed"] = {"steps_per_s": mixed["steps_per_s"],
+                         "speedup_vs_fp64": mixed["steps_per_s"] / fp64["steps_per_s"],
+                         "kernels_ms": {n: round(k["avg_ms"], 5) for n, k in mixed["kernels"].items()},
+                         "energy_rel_err_vs_fp64": abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"]),
+                         "finite": mixed["finite"]}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(ic, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -100,7 +100,9 @@ const char *mwf_last_error(void);
  *   scl * exp(-(kx^2 + kz^2)/k0^2) * (N(0,1) + i N(0,1))/sqrt(2),
  * kx = alpha*m (signed), kz = gamma*n; projected per wavenumber onto the
  * divergence-free toroidal/poloidal form; zero deviation at (0,0) (laminar
- * mean); n = 0 column made Hermitian.  Writes an mpt in the layout above. */
+ * mean); n = 0 column made Hermitian; finally scaled so that the
+ * perturbation energy (vel_energy, the mean square velocity) is scl^2.
+ * Writes an mpt in the layout above. */
 int mwf_ic_gaussian(const mwf_params *p, uint64_t seed, double scl, double k0,
                     double *re, double *im);
 

@@ -15,6 +15,11 @@
 //     with kh^2 = kx^2 + kz^2 and D_k = ad_k1(k) = (-1)^k k beta.
 //  4. Zero deviation at (m,n) = (0,0): the mean flow is the laminar profile.
 //  5. n = 0 column made Hermitian: c(M-m, 0) = conj(c(m, 0)).
+//  6. Scaled so that the perturbation energy of vel_energy (the mean square
+//     velocity, program/velocity.f90:115-162) equals scl^2: scl is the rms
+//     velocity of the perturbation and A(m,n) only shapes its spectrum.
+//     (Unscaled, every mode at amplitude ~scl puts 10^2-10^4 times the
+//     laminar energy into the box and the step diverges within tens of steps.)
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -43,6 +48,7 @@ extern "C" int mwf_ic_gaussian(const mwf_params* p, uint64_t seed, double scl, d
     const double beta = 3.1415926535897931 / 2.0, twopi = 6.283185307179586476925286766559;
     SplitMix64 rng{seed};
     std::vector<double> sr(KK), si(KK);
+    double esum = 0.0; // Parseval energy of the projected field
     std::memset(re, 0, sizeof(double) * (size_t)K * M * NN);
     std::memset(im, 0, sizeof(double) * (size_t)K * M * NN);
     for (int n = 0; n < NN; n++)
@@ -66,7 +72,16 @@ extern "C" int mwf_ic_gaussian(const mwf_params* p, uint64_t seed, double scl, d
                 const size_t ip = (size_t)k + (size_t)K * ((size_t)mi + (size_t)M * n);
                 re[ip] = psr;
                 im[ip] = psi;
-                if (k == 0) continue;
+                // energy weights: 2 for n > 0 (the -n half), m > 0 only at n = 0
+                // (the column is made Hermitian below); y weight 1 for k = 0
+                const double wn = n > 0 ? 2.0 : (m > 0 ? 2.0 : (m == 0 ? 1.0 : 0.0));
+                const double yw = k == 0 ? 1.0 : 0.5;
+                // u = -i kz psi, w = i kx psi (+ phi parts below)
+                double ur = kz * psi, ui = -kz * psr, wr = -kx * psi, wi = kx * psr, vr = 0.0, vi = 0.0;
+                if (k == 0) {
+                    esum += wn * yw * (ur * ur + ui * ui + wr * wr + wi * wi);
+                    continue;
+                }
                 const double D = ((k & 1) ? -1.0 : 1.0) * k * beta;
                 // b^H s = -i kx D u + kh2 v - i kz D w
                 const double tr = kx * D * si[cu] + kh2 * sr[cv] + kz * D * si[cw];
@@ -75,8 +90,23 @@ extern "C" int mwf_ic_gaussian(const mwf_params* p, uint64_t seed, double scl, d
                 const size_t iph = (size_t)(K0 - 1 + k) + (size_t)K * ((size_t)mi + (size_t)M * n);
                 re[iph] = tr / den;
                 im[iph] = ti / den;
+                const double fr = tr / den, fi = ti / den;
+                ur += -kx * D * fi;
+                ui += kx * D * fr;
+                wr += -kz * D * fi;
+                wi += kz * D * fr;
+                vr = kh2 * fr;
+                vi = kh2 * fi;
+                esum += wn * yw * (ur * ur + ui * ui + vr * vr + vi * vi + wr * wr + wi * wi);
             }
         }
+    if (esum > 0.0) {
+        const double f = scl / std::sqrt(esum);
+        for (size_t i = 0; i < (size_t)K * M * NN; i++) {
+            re[i] *= f;
+            im[i] *= f;
+        }
+    }
     // Hermitian n = 0 column
     for (int m = 1; m < MM; m++)
         for (int k = 0; k < K; k++) {
