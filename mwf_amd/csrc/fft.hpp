@@ -3,19 +3,28 @@
 // The 3/2-padded transforms of the reference (lengths P = 3*MM, Q = 3*NN,
 // program/transform.fftw5.f90:33-59) are split by the kernels into a radix-3
 // fold done in registers and three power-of-two transforms of length MM (or
-// NN) done here.  A block transforms S sequences stored back to back in LDS
-// (sequence s at buf[s*L]).  Each pass is a Stockham radix-R step:
-// every butterfly reads R points L/R apart, applies the twiddles of the
-// current sub-transform, does a radix-R DFT in registers and writes the R
-// results Ns apart, so the output comes out in natural order.  Passes run in
-// place: a block first loads all its butterflies into registers, then
-// synchronises, then writes.
+// NN) done here.  A block transforms S sequences stored back to back in LDS.
+// Each pass is a Stockham radix-R step: every butterfly reads R points L/R
+// apart, applies the twiddles of the current sub-transform, does a radix-R
+// DFT in registers and writes the R results Ns apart, so the output comes out
+// in natural order.
+//
+// LDS layout: sequence s starts at s*L and element i sits at the swizzled
+// slot pidx(i) = i ^ ((i >> 3) & 15), a permutation within each aligned block
+// of 128.  Every read and write pattern of these passes then hits distinct
+// 16-byte bank groups within each 8- or 16-lane group of a b128 access (and
+// each 16- or 32-lane group of a b64 access), so the passes run free of bank
+// conflicts without padding.
 //
 // Transforms are unnormalised: sign +1 computes sum_j x_j exp(+2 pi i jk/L)
 // (FFTW's backward), sign -1 exp(-2 pi i jk/L).  Twiddles come from a table
-// tw[j] = exp(+2 pi i j / L) rounded from long double on the host.
+// tw[j] = exp(+2 pi i j / L) rounded from long double on the host; a radix-8
+// butterfly loads w, w^2, w^4 and forms w^3, w^5, w^6, w^7 by products.
 #pragma once
 #include "common.hpp"
+
+__host__ __device__ constexpr int lpad(int L) { return L; }
+__device__ __forceinline__ int pidx(int i) { return i ^ ((i >> 3) & 15); }
 
 template <int SIGN, typename CT> __device__ __forceinline__ void dft2(CT* x)
 {
@@ -65,58 +74,112 @@ template <int R, int SIGN, typename CT> __device__ __forceinline__ void dftR(CT*
     else dft8<SIGN>(x);
 }
 
-// One in-place Stockham pass over S sequences of length L.
-template <typename CT, int L, int R, int Ns, int SIGN, int T, int MAXB>
-__device__ __forceinline__ void stockham_pass(CT* __restrict__ buf, int S, const CT* __restrict__ tw)
+// x[r] *= w^r, w = tw[e] (conjugated for SIGN < 0)
+template <int R, int SIGN, typename CT>
+__device__ __forceinline__ void twiddle(CT* x, const CT* __restrict__ tw, int e)
 {
-    constexpr int NB = L / R;
-    const int total = S * NB;
-    CT x[MAXB][R];
-#pragma unroll
-    for (int q = 0; q < MAXB; q++) {
-        const int i = threadIdx.x + q * T;
-        if (i < total) {
-            const int s = i / NB, j = i % NB;
-            const CT* b = buf + s * L;
-#pragma unroll
-            for (int r = 0; r < R; r++) x[q][r] = b[j + r * NB];
-        }
+    auto ld = [&](int i) {
+        CT w = tw[i];
+        if (SIGN < 0) w = cconj(w);
+        return w;
+    };
+    if constexpr (R == 2) {
+        x[1] = cmul(x[1], ld(e));
+    } else if constexpr (R == 4) {
+        const CT w1 = ld(e), w2 = ld(2 * e);
+        x[1] = cmul(x[1], w1);
+        x[2] = cmul(x[2], w2);
+        x[3] = cmul(x[3], cmul(w1, w2));
+    } else {
+        const CT w1 = ld(e), w2 = ld(2 * e), w4 = ld(4 * e);
+        const CT w3 = cmul(w1, w2);
+        x[1] = cmul(x[1], w1);
+        x[2] = cmul(x[2], w2);
+        x[3] = cmul(x[3], w3);
+        x[4] = cmul(x[4], w4);
+        x[5] = cmul(x[5], cmul(w1, w4));
+        x[6] = cmul(x[6], cmul(w2, w4));
+        x[7] = cmul(x[7], cmul(w3, w4));
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < MAXB; q++) {
-        const int i = threadIdx.x + q * T;
-        if (i < total) {
-            const int s = i / NB, j = i % NB;
-            const int k = j & (Ns - 1);
-            if constexpr (Ns > 1) {
-#pragma unroll
-                for (int r = 1; r < R; r++) {
-                    CT w = ld_tw(tw, r * k * (L / (Ns * R)));
-                    if (SIGN < 0) w = cconj(w);
-                    x[q][r] = cmul(x[q][r], w);
-                }
-            }
-            dftR<R, SIGN>(x[q]);
-            CT* b = buf + s * L;
-            const int o = (j - k) * R + k;
-#pragma unroll
-            for (int r = 0; r < R; r++) b[o + r * Ns] = x[q][r];
-        }
-    }
-    __syncthreads();
 }
 
-// All passes: radix 8 while at least 8 remain, then 4 or 2.
-template <typename CT, int LOG, int SIGN, int T, int SMAX, int Ns = 1>
-__device__ __forceinline__ void lds_fft(CT* __restrict__ buf, int S, const CT* __restrict__ tw)
+// One radix-R butterfly j of sequence base (load, twiddle, DFT); outputs stay in x.
+template <typename CT, int L, int R, int Ns, int SIGN>
+__device__ __forceinline__ void bfly_load(const CT* __restrict__ sb, int j, CT* x, const CT* __restrict__ tw)
 {
-    constexpr int L = 1 << LOG;
+    constexpr int NB = L / R;
+#pragma unroll
+    for (int r = 0; r < R; r++) x[r] = sb[pidx(j + r * NB)];
+    if constexpr (Ns > 1) twiddle<R, SIGN>(x, tw, (j & (Ns - 1)) * (L / (Ns * R)));
+    dftR<R, SIGN>(x);
+}
+
+template <typename CT, int R, int Ns>
+__device__ __forceinline__ void bfly_store(CT* __restrict__ db, int j, const CT* x)
+{
+    const int k = j & (Ns - 1);
+    const int o = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) db[pidx(o + r * Ns)] = x[r];
+}
+
+template <int L, int Ns> constexpr int radix_at() { return (L / Ns) >= 8 ? 8 : (L / Ns); }
+
+template <int L, int Ns = 1> constexpr int npasses()
+{
+    if constexpr (Ns >= L) return 0;
+    else return 1 + npasses<L, Ns * radix_at<L, Ns>()>();
+}
+
+// Ping-pong FFT over S sequences: pass 0 reads A and writes B, pass 1 reads B
+// and writes A, ...  Ends with a barrier.  The result is in B when the pass
+// count is odd (fft_in_b<LOG>()), else in A.
+template <typename CT, int LOG, int SIGN, int T, int Ns = 1, int PASS = 0>
+__device__ __forceinline__ void pp_fft(CT* A, CT* B, int S, const CT* __restrict__ tw)
+{
+    constexpr int L = 1 << LOG, LP = lpad(L);
     if constexpr (Ns < L) {
-        constexpr int rem = L / Ns;
-        constexpr int R = rem >= 8 ? 8 : rem;
-        constexpr int MAXB = (SMAX * (L / R) + T - 1) / T;
-        stockham_pass<CT, L, R, Ns, SIGN, T, MAXB>(buf, S, tw);
-        lds_fft<CT, LOG, SIGN, T, SMAX, Ns * R>(buf, S, tw);
+        constexpr int R = radix_at<L, Ns>();
+        constexpr int NB = L / R;
+        const CT* src = (PASS & 1) ? B : A;
+        CT* dst = (PASS & 1) ? A : B;
+        const int total = S * NB;
+        for (int i = threadIdx.x; i < total; i += T) {
+            const int s = i / NB, j = i % NB;
+            CT x[R];
+            bfly_load<CT, L, R, Ns, SIGN>(src + s * LP, j, x, tw);
+            bfly_store<CT, R, Ns>(dst + s * LP, j, x);
+        }
+        __syncthreads();
+        pp_fft<CT, LOG, SIGN, T, Ns * R, PASS + 1>(A, B, S, tw);
+    }
+}
+template <int LOG> constexpr bool fft_in_b() { return (npasses<(1 << LOG)>() & 1) != 0; }
+
+// In-place FFT (all butterflies of a pass are loaded before any is stored);
+// for kernels whose LDS cannot hold two buffers.  SMAX bounds S.
+template <typename CT, int LOG, int SIGN, int T, int SMAX, int Ns = 1>
+__device__ __forceinline__ void ip_fft(CT* buf, int S, const CT* __restrict__ tw)
+{
+    constexpr int L = 1 << LOG, LP = lpad(L);
+    if constexpr (Ns < L) {
+        constexpr int R = radix_at<L, Ns>();
+        constexpr int NB = L / R;
+        constexpr int MAXB = (SMAX * NB + T - 1) / T;
+        const int total = S * NB;
+        CT x[MAXB][R];
+#pragma unroll
+        for (int q = 0; q < MAXB; q++) {
+            const int i = threadIdx.x + q * T;
+            if (i < total) bfly_load<CT, L, R, Ns, SIGN>(buf + (i / NB) * LP, i % NB, x[q], tw);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MAXB; q++) {
+            const int i = threadIdx.x + q * T;
+            if (i < total) bfly_store<CT, R, Ns>(buf + (i / NB) * LP, i % NB, x[q]);
+        }
+        __syncthreads();
+        ip_fft<CT, LOG, SIGN, T, SMAX, Ns * R>(buf, S, tw);
     }
 }

@@ -87,6 +87,7 @@ __global__ __launch_bounds__(XT) void k_xinv(Geo g, const double2* __restrict__ 
 {
     using CT = cplx_t<R>;
     constexpr int MM = 1 << LOGMM;
+    constexpr int LP = lpad(MM);
     constexpr int IPT = XTILE / XT; // items per thread
     extern __shared__ unsigned char smem_raw[];
     CT* buf = reinterpret_cast<CT*>(smem_raw);
@@ -183,19 +184,19 @@ __global__ __launch_bounds__(XT) void k_xinv(Geo g, const double2* __restrict__ 
                 xwave(g, MM - 1 + r, am1, am2);
                 X2 = double2{-(X2.y * am1), X2.x * am1};
             }
-            CT* bt = buf + (size_t)t * 3 * MM;
-            bt[r] = CT{(R)(X1.x + X2.x), (R)(X1.y + X2.y)};
+            CT* bt = buf + (size_t)t * 3 * LP;
+            bt[pidx(r)] = CT{(R)(X1.x + X2.x), (R)(X1.y + X2.y)};
 #pragma unroll
             for (int b = 1; b < 3; b++) {
                 const double2 w1 = twP[r * b];
                 const double2 w2 = twP[(MM - r) * b]; // conj: exp(2 pi i (r-MM) b / P)
                 const double y_r = (X1.x * w1.x - X1.y * w1.y) + (X2.x * w2.x + X2.y * w2.y);
                 const double y_i = (X1.x * w1.y + X1.y * w1.x) + (X2.y * w2.x - X2.x * w2.y);
-                bt[b * MM + r] = CT{(R)y_r, (R)y_i};
+                bt[b * LP + pidx(r)] = CT{(R)y_r, (R)y_i};
             }
         }
         __syncthreads();
-        lds_fft<CT, LOGMM, +1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
+        ip_fft<CT, LOGMM, +1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
         // write U[x][line][n0 + t]
 #pragma unroll
         for (int q = 0; q < 3 * IPT; q++) {
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(XT) void k_xinv(Geo g, const double2* __restrict__ 
             if (it >= 3 * nitems) continue;
             const int x = it / NT, t = it - x * NT;
             const int a = x / 3, b = x - 3 * a;
-            U[((size_t)x * 2 * g.KK + line) * g.NN + n0 + t] = buf[((size_t)t * 3 + b) * MM + a];
+            U[((size_t)x * 2 * g.KK + line) * g.NN + n0 + t] = buf[((size_t)t * 3 + b) * LP + pidx(a)];
         }
         __syncthreads();
     }
@@ -211,16 +212,23 @@ __global__ __launch_bounds__(XT) void k_xinv(Geo g, const double2* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // z fused: block of XB = ZT/NN x-rows; thread (row xl, a) owns the physical
-// points z = 3a + b, b = 0..2 of its row.
+// points z = 3a + b, b = 0..2 of its row.  The 3*KK input rows (u, du/dx,
+// du/dz) are taken in batches of ZP pairs.  A fold task (row, pair, r) with
+// r in [0, NN/2) holds X[r] and X[NN-r] of both rows of the pair and writes
+// the folded spectra at r and NN-r (r = 0 covers 0 and NN/2); its four values
+// for the next batch are loaded from HBM while the current batch is
+// transformed and consumed.
 // ---------------------------------------------------------------------------
+template <typename R> constexpr int zpairs() { return sizeof(R) == 8 ? 2 : 4; }
+
 template <typename R, int K0, int LOGNN>
-__device__ __forceinline__ cplx_t<R> zload(const Geo& g, const cplx_t<R>* __restrict__ Urow, int gi, int n)
+__device__ __forceinline__ cplx_t<R> zload_rt(const Geo& g, const cplx_t<R>* __restrict__ Urow, int gi, int n)
 {
     using CT = cplx_t<R>;
     constexpr int KK = 3 * K0 - 1;
+    constexpr int NN = 1 << LOGNN;
     if (gi >= 3 * KK) return CT{(R)0, (R)0};
     const int fam = gi / KK, s = gi - fam * KK;
-    constexpr int NN = 1 << LOGNN;
     if (fam == 1) return Urow[(KK + s) * NN + n];
     CT v = Urow[s * NN + n];
     if (fam == 2) { // i gamma n (program/variables.f90:608-609)
@@ -228,6 +236,12 @@ __device__ __forceinline__ cplx_t<R> zload(const Geo& g, const cplx_t<R>* __rest
         v = CT{-(v.y * an1), v.x * an1};
     }
     return v;
+}
+
+template <typename R, int K0, int LOGNN> constexpr size_t z_lds_bytes()
+{
+    constexpr int NN = 1 << LOGNN, XB = ZT / NN, ZP = zpairs<R>();
+    return sizeof(cplx_t<R>) * 2 * (size_t)XB * ZP * 3 * lpad(NN);
 }
 
 template <typename R, int K0, int LOGNN>
@@ -239,20 +253,47 @@ __global__ __launch_bounds__(ZT) void k_z(Geo g, const cplx_t<R>* __restrict__ U
     constexpr int KK = 3 * K0 - 1;
     constexpr int NN = 1 << LOGNN;
     constexpr int XB = ZT / NN;
-    constexpr int NC = 3 * KK;                             // u, du/dx, du/dz
-    constexpr int NBATCH = (NC + 2 * ZPB - 1) / (2 * ZPB); // inverse batches
-    constexpr int NPA = (KK + 1) / 2;                      // output pairs
-    constexpr int NROUND = (NPA + ZPB - 1) / ZPB;          // forward rounds
-    constexpr int SEQ = XB * ZPB * 3;
+    constexpr int NC = 3 * KK;                      // u, du/dx, du/dz
+    constexpr int ZP = zpairs<R>();                 // pairs per batch
+    constexpr int NPI = (NC + 1) / 2, NBI = (NPI + ZP - 1) / ZP;
+    constexpr int NPO = (KK + 1) / 2, NBO = (NPO + ZP - 1) / ZP;
+    constexpr int SEQ = XB * ZP * 3;
+    constexpr int LP = lpad(NN);
+    constexpr int H = NN / 2;
+    constexpr int FT = XB * ZP * H / ZT;            // fold tasks per thread
+    constexpr bool INB = fft_in_b<LOGNN>();
+    static_assert(XB * ZP * H % ZT == 0, "fold tasks must tile the block");
     extern __shared__ unsigned char smem_raw[];
-    CT* buf = reinterpret_cast<CT*>(smem_raw);
+    CT* A = reinterpret_cast<CT*>(smem_raw);        // [SEQ][LP]
+    CT* B = A + SEQ * LP;
 
     const int tid = threadIdx.x;
     const int xl = tid >> LOGNN, r = tid & (NN - 1);
-    const int x = blockIdx.x * XB + xl;
+    const int x0 = blockIdx.x * XB;
+    const int x = x0 + xl;
     const bool valid = x < g.P;
-    const CT* Urow = U + (size_t)(valid ? x : 0) * 2 * KK * NN;
-    const int Q = 3 * NN;
+
+    // fold task f of this thread: tau = tid + ZT*f -> (row fx, pair fp, index fr)
+    CT pf[FT][4];
+    auto prefetch = [&](int batch) {
+#pragma unroll
+        for (int f = 0; f < FT; f++) {
+            const int tau = tid + ZT * f;
+            const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
+            const int xg = x0 + fx;
+            const int g1 = batch * 2 * ZP + 2 * fp;
+            const int r2 = fr == 0 ? H : NN - fr;
+            if (xg < g.P) {
+                const CT* Urow = U + (size_t)xg * 2 * KK * NN;
+                pf[f][0] = zload_rt<R, K0, LOGNN>(g, Urow, g1, fr);
+                pf[f][1] = zload_rt<R, K0, LOGNN>(g, Urow, g1, r2);
+                pf[f][2] = zload_rt<R, K0, LOGNN>(g, Urow, g1 + 1, fr);
+                pf[f][3] = zload_rt<R, K0, LOGNN>(g, Urow, g1 + 1, r2);
+            } else {
+                pf[f][0] = pf[f][1] = pf[f][2] = pf[f][3] = CT{(R)0, (R)0};
+            }
+        }
+    };
 
     R vb[3][KK];
     R an[3][KK];
@@ -261,49 +302,61 @@ __global__ __launch_bounds__(ZT) void k_z(Geo g, const cplx_t<R>* __restrict__ U
 #pragma unroll
         for (int s = 0; s < KK; s++) an[b][s] = (R)0;
 
-    static_for<0, NBATCH>([&](auto bi) {
-        constexpr int B = decltype(bi)::value;
-        // fold two real rows per complex sequence
-        static_for<0, ZPB>([&](auto pi) {
-            constexpr int p = decltype(pi)::value;
-            constexpr int g1 = B * 2 * ZPB + 2 * p, g2 = g1 + 1;
-            CT Z[3];
-            if (valid && g1 < NC) {
-                const CT a1 = zload<R, K0, LOGNN>(g, Urow, g1, r);
-                const CT a2 = zload<R, K0, LOGNN>(g, Urow, g2, r);
-                if (r == 0) {
-                    Z[0] = Z[1] = Z[2] = CT{a1.x, a2.x};
-                } else {
-                    const CT b1 = zload<R, K0, LOGNN>(g, Urow, g1, NN - r);
-                    const CT b2 = zload<R, K0, LOGNN>(g, Urow, g2, NN - r);
-                    Z[0] = CT{(a1.x + b1.x) - (a2.y - b2.y), (a1.y - b1.y) + (a2.x + b2.x)};
+    prefetch(0);
+    static_for<0, NBI>([&](auto bi) {
+        constexpr int BN = decltype(bi)::value;
+        // fold: Y_b[s] = X[s] w^{sb} + conj(X[NN-s]) w^{-(NN-s)b}, w = exp(2 pi i/Q),
+        // two real rows per complex sequence Z = Y(row 1) + i Y(row 2)
 #pragma unroll
-                    for (int b = 1; b < 3; b++) {
-                        const CT w1 = twQ[r * b];
-                        const CT w2 = twQ[(NN - r) * b]; // conj -> exp(-2 pi i (NN-r) b / Q)
-                        // Y = a w1 + conj(b) conj(w2) = a w1 + conj(b w2)
-                        const CT bw1 = cmul(b1, w2), bw2 = cmul(b2, w2);
-                        const CT y1 = cadd(cmul(a1, w1), cconj(bw1));
-                        const CT y2 = cadd(cmul(a2, w1), cconj(bw2));
-                        Z[b] = CT{y1.x - y2.y, y1.y + y2.x};
-                    }
+        for (int f = 0; f < FT; f++) {
+            const int tau = tid + ZT * f;
+            const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
+            CT* seq = A + ((fx * ZP + fp) * 3) * LP;
+            const CT a1 = pf[f][0], b1 = pf[f][1], a2 = pf[f][2], b2 = pf[f][3];
+            if (fr == 0) {
+                // s = 0: the imaginary part of the DC bin is dropped (c2r);
+                // s = NN/2 is its own mirror: Y = 2 Re(X w^{(NN/2) b})
+#pragma unroll
+                for (int b = 0; b < 3; b++) seq[b * LP + pidx(0)] = CT{a1.x, a2.x};
+                seq[pidx(H)] = CT{(R)2 * b1.x, (R)2 * b2.x};
+#pragma unroll
+                for (int b = 1; b < 3; b++) {
+                    const CT w = twQ[H * b];
+                    const CT p1 = cmul(b1, w), p2 = cmul(b2, w);
+                    seq[b * LP + pidx(H)] = CT{(R)2 * p1.x, (R)2 * p2.x};
                 }
             } else {
-                Z[0] = Z[1] = Z[2] = CT{(R)0, (R)0};
-            }
+                const int fm = NN - fr;
 #pragma unroll
-            for (int b = 0; b < 3; b++) buf[((xl * ZPB + p) * 3 + b) * NN + r] = Z[b];
-        });
+                for (int b = 0; b < 3; b++) {
+                    CT P1a = a1, P2a = b1, P1b = a2, P2b = b2; // X[s] w^{sb}, X[NN-s] w^{(NN-s)b}
+                    if (b > 0) {
+                        const CT w1 = twQ[fr * b], w2 = twQ[fm * b];
+                        P1a = cmul(a1, w1);
+                        P2a = cmul(b1, w2);
+                        P1b = cmul(a2, w1);
+                        P2b = cmul(b2, w2);
+                    }
+                    // row 1: Y[fr] = P1a + conj(P2a), Y[fm] = P2a + conj(P1a); row 2 likewise
+                    const CT y1r = CT{P1a.x + P2a.x, P1a.y - P2a.y}, y1m = CT{P2a.x + P1a.x, P2a.y - P1a.y};
+                    const CT y2r = CT{P1b.x + P2b.x, P1b.y - P2b.y}, y2m = CT{P2b.x + P1b.x, P2b.y - P1b.y};
+                    seq[b * LP + pidx(fr)] = CT{y1r.x - y2r.y, y1r.y + y2r.x};
+                    seq[b * LP + pidx(fm)] = CT{y1m.x - y2m.y, y1m.y + y2m.x};
+                }
+            }
+        }
+        if constexpr (BN + 1 < NBI) prefetch(BN + 1);
         __syncthreads();
-        lds_fft<CT, LOGNN, +1, ZT, SEQ>(buf, SEQ, twNN);
+        pp_fft<CT, LOGNN, +1, ZT>(A, B, SEQ, twNN);
+        const CT* F = INB ? B : A;
         // consume: thread (xl, a = r)
-        static_for<0, ZPB>([&](auto pi) {
+        static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
 #pragma unroll
             for (int b = 0; b < 3; b++) {
-                const CT zv = buf[((xl * ZPB + p) * 3 + b) * NN + r];
+                const CT zv = F[((xl * ZP + p) * 3 + b) * LP + pidx(r)];
                 static_for<0, 2>([&](auto hi) {
-                    constexpr int gi = B * 2 * ZPB + 2 * p + decltype(hi)::value;
+                    constexpr int gi = BN * 2 * ZP + 2 * p + decltype(hi)::value;
                     if constexpr (gi < NC) {
                         const R val = decltype(hi)::value == 0 ? zv.x : zv.y;
                         constexpr int fam = gi / KK, s = gi % KK;
@@ -314,39 +367,40 @@ __global__ __launch_bounds__(ZT) void k_z(Geo g, const cplx_t<R>* __restrict__ U
                 });
             }
         });
-        __syncthreads();
+        if constexpr (!INB) __syncthreads();
     });
 #pragma unroll
     for (int b = 0; b < 3; b++) G::yterms(vb[b], an[b], (R)g.beta);
 
     // forward z transform of the KK products, two per complex sequence
     const R scale = (R)g.scale;
-    static_for<0, NROUND>([&](auto ri) {
+    static_for<0, NBO>([&](auto ri) {
         constexpr int RD = decltype(ri)::value;
-        static_for<0, ZPB>([&](auto pi) {
+        static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
-            constexpr int c1 = 2 * (RD * ZPB + p), c2 = c1 + 1;
+            constexpr int c1 = 2 * (RD * ZP + p), c2 = c1 + 1;
             if constexpr (c1 < KK) {
 #pragma unroll
                 for (int b = 0; b < 3; b++) {
                     R v2 = (R)0;
                     if constexpr (c2 < KK) v2 = an[b][c2];
-                    buf[((xl * ZPB + p) * 3 + b) * NN + r] = CT{an[b][c1], v2};
+                    A[((xl * ZP + p) * 3 + b) * LP + pidx(r)] = CT{an[b][c1], v2};
                 }
             }
         });
         __syncthreads();
-        lds_fft<CT, LOGNN, -1, ZT, SEQ>(buf, SEQ, twNN);
-        static_for<0, ZPB>([&](auto pi) {
+        pp_fft<CT, LOGNN, -1, ZT>(A, B, SEQ, twNN);
+        const CT* F = INB ? B : A;
+        static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
-            constexpr int c1 = 2 * (RD * ZPB + p), c2 = c1 + 1;
+            constexpr int c1 = 2 * (RD * ZP + p), c2 = c1 + 1;
             if constexpr (c1 < KK) {
                 CT X1{(R)0, (R)0}, X2{(R)0, (R)0};
                 const int rm = (NN - r) & (NN - 1);
 #pragma unroll
                 for (int b = 0; b < 3; b++) {
-                    const CT zn = buf[((xl * ZPB + p) * 3 + b) * NN + r];
-                    const CT zm = buf[((xl * ZPB + p) * 3 + b) * NN + rm];
+                    const CT zn = F[((xl * ZP + p) * 3 + b) * LP + pidx(r)];
+                    const CT zm = F[((xl * ZP + p) * 3 + b) * LP + pidx(rm)];
                     const CT F1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
                     const CT F2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
                     if (b == 0) {
@@ -365,7 +419,7 @@ __global__ __launch_bounds__(ZT) void k_z(Geo g, const cplx_t<R>* __restrict__ U
                 }
             }
         });
-        __syncthreads();
+        if constexpr (!INB) __syncthreads();
     });
 }
 
@@ -380,6 +434,7 @@ __global__ __launch_bounds__(ZT) void k_zphys(Geo g, const cplx_t<R>* __restrict
     constexpr int XB = ZT / NN;
     constexpr int NBATCH = (KK + 2 * ZPB - 1) / (2 * ZPB);
     constexpr int SEQ = XB * ZPB * 3;
+    constexpr int LP = lpad(NN);
     extern __shared__ unsigned char smem_raw[];
     CT* buf = reinterpret_cast<CT*>(smem_raw);
     const int tid = threadIdx.x;
@@ -412,13 +467,13 @@ __global__ __launch_bounds__(ZT) void k_zphys(Geo g, const cplx_t<R>* __restrict
             } else {
                 Z[0] = Z[1] = Z[2] = CT{(R)0, (R)0};
             }
-            for (int b = 0; b < 3; b++) buf[((xl * ZPB + p) * 3 + b) * NN + r] = Z[b];
+            for (int b = 0; b < 3; b++) buf[((xl * ZPB + p) * 3 + b) * LP + pidx(r)] = Z[b];
         }
         __syncthreads();
-        lds_fft<CT, LOGNN, +1, ZT, SEQ>(buf, SEQ, twNN);
+        ip_fft<CT, LOGNN, +1, ZT, SEQ>(buf, SEQ, twNN);
         for (int p = 0; p < ZPB; p++) {
             for (int b = 0; b < 3; b++) {
-                const CT zv = buf[((xl * ZPB + p) * 3 + b) * NN + r];
+                const CT zv = buf[((xl * ZPB + p) * 3 + b) * LP + pidx(r)];
                 const int g1 = B * 2 * ZPB + 2 * p;
                 const int z = 3 * r + b;
                 if (valid && g1 < KK) phys[((size_t)x * Q + z) * KK + g1] = (double)zv.x;
@@ -506,6 +561,7 @@ __global__ __launch_bounds__(XT) void k_xfwd(Geo g, const cplx_t<R>* __restrict_
 {
     using CT = cplx_t<R>;
     constexpr int MM = 1 << LOGMM;
+    constexpr int LP = lpad(MM);
     constexpr int IPT = XTILE / XT;
     extern __shared__ unsigned char smem_raw[];
     CT* buf = reinterpret_cast<CT*>(smem_raw);
@@ -530,25 +586,25 @@ __global__ __launch_bounds__(XT) void k_xfwd(Geo g, const cplx_t<R>* __restrict_
             if (it >= 3 * nitems) continue;
             const int x = it / NT, t = it - x * NT;
             const int a = x / 3, b = x - 3 * a;
-            buf[((size_t)t * 3 + b) * MM + a] = NL[((size_t)x * g.KK + slot) * g.NN + n0 + t];
+            buf[((size_t)t * 3 + b) * LP + pidx(a)] = NL[((size_t)x * g.KK + slot) * g.NN + n0 + t];
         }
         __syncthreads();
-        lds_fft<CT, LOGMM, -1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
+        ip_fft<CT, LOGMM, -1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
 #pragma unroll
         for (int q = 0; q < IPT; q++) {
             const int it = tid + q * XT;
             if (it >= nitems) continue;
             const int t = it >> LOGMM, r = it & (MM - 1);
             const int n = n0 + t;
-            const CT* bt = buf + (size_t)t * 3 * MM;
+            const CT* bt = buf + (size_t)t * 3 * LP;
             double2 Xp, Xn;
             {
-                const CT F0 = bt[r];
+                const CT F0 = bt[pidx(r)];
                 Xp = double2{(double)F0.x, (double)F0.y};
                 Xn = Xp;
 #pragma unroll
                 for (int b = 1; b < 3; b++) {
-                    const CT Fb = bt[b * MM + r];
+                    const CT Fb = bt[b * LP + pidx(r)];
                     const double fr = (double)Fb.x, fi = (double)Fb.y;
                     const double2 w1 = twP[r * b];        // m = r:      times conj(w1)
                     const double2 w2 = twP[(MM - r) * b]; // m = r - MM: times w2

@@ -50,7 +50,7 @@ static hipError_t launch_xinv(const Geo& g, hipStream_t s, const double2* c, voi
                               const void* twMM, const double2* twP, int addlam)
 {
     using CT = cplx_t<R>;
-    const size_t lds = 3 * (size_t)XTILE * sizeof(CT);
+    const size_t lds = 3 * (size_t)(XTILE >> LOGMM) * lpad(1 << LOGMM) * sizeof(CT);
     set_lds(k_xinv<R, LOGMM>, lds);
     dim3 grid(g.NN / g.NTx, g.K0);
     hipLaunchKernelGGL((k_xinv<R, LOGMM>), grid, dim3(XT), lds, s, g, c, (CT*)U, ep, (const CT*)twMM, twP,
@@ -63,7 +63,7 @@ static hipError_t launch_xfwd(const Geo& g, hipStream_t s, const void* NL, doubl
                               const void* twMM, const double2* twP, int mode, int nst)
 {
     using CT = cplx_t<R>;
-    const size_t lds = 3 * (size_t)XTILE * sizeof(CT);
+    const size_t lds = 3 * (size_t)(XTILE >> LOGMM) * lpad(1 << LOGMM) * sizeof(CT);
     set_lds(k_xfwd<R, LOGMM>, lds);
     dim3 grid(g.NN / g.NTx, g.K0);
     hipLaunchKernelGGL((k_xfwd<R, LOGMM>), grid, dim3(XT), lds, s, g, (const CT*)NL, c, onl, (const CT*)twMM,
@@ -77,7 +77,7 @@ static hipError_t launch_z(const Geo& g, hipStream_t s, const void* U, void* NL,
 {
     using CT = cplx_t<R>;
     constexpr int XB = ZT >> LOGNN;
-    const size_t lds = (size_t)XB * ZPB * 3 * (1 << LOGNN) * sizeof(CT);
+    const size_t lds = z_lds_bytes<R, K0, LOGNN>();
     set_lds(k_z<R, K0, LOGNN>, lds);
     dim3 grid((g.P + XB - 1) / XB);
     hipLaunchKernelGGL((k_z<R, K0, LOGNN>), grid, dim3(ZT), lds, s, g, (const CT*)U, (CT*)NL,
@@ -91,7 +91,7 @@ static hipError_t launch_zphys(const Geo& g, hipStream_t s, const void* U, doubl
 {
     using CT = cplx_t<R>;
     constexpr int XB = ZT >> LOGNN;
-    const size_t lds = (size_t)XB * ZPB * 3 * (1 << LOGNN) * sizeof(CT);
+    const size_t lds = (size_t)XB * ZPB * 3 * lpad(1 << LOGNN) * sizeof(CT);
     set_lds(k_zphys<R, K0, LOGNN>, lds);
     dim3 grid((g.P + XB - 1) / XB);
     hipLaunchKernelGGL((k_zphys<R, K0, LOGNN>), grid, dim3(ZT), lds, s, g, (const CT*)U, phys,

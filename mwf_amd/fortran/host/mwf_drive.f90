@@ -1,0 +1,54 @@
+!
+! mwf_drive.f90 -- driver with the time loop of program/main.f90:20-27 over
+! the GPU velocity variant: diagnostics before each step (energy every
+! i_save_rate2 steps to vel_energy.dat as '(2e20.12)', program/io.f90:460),
+! then vel_TS, then t += dt.  The initial state comes from the seeded
+! Gaussian generator of libmwf (mwf_ic_gaussian); the final state is written
+! to final_state.bin (Re then Im, raw doubles) after vel_sync_host.
+!   usage: mwf_drive <nsteps> <seed> <rms> <k0>
+!
+program mwf_drive
+   use iso_c_binding
+   use parameters
+   use variables
+   use velocity
+   use mwf_capi
+   implicit none
+   type(phys) :: vp
+   type(mwf_params) :: p
+   double precision :: E, scl, k0
+   integer :: nsteps, io_KE, uo
+   integer(c_int64_t) :: seed
+   character(64) :: arg
+
+   call get_command_argument(1, arg); read(arg, *) nsteps
+   call get_command_argument(2, arg); read(arg, *) seed
+   call get_command_argument(3, arg); read(arg, *) scl
+   call get_command_argument(4, arg); read(arg, *) k0
+
+   tim_t = 0d0
+   tim_step = 0
+   call vel_precompute()
+   p%k0 = i_K0; p%mm = i_MM; p%nn = i_NN; p%dragall = 1; p%precision = MWF_FP64; p%reserved = 0
+   p%alpha = d_alpha; p%gamma = d_gamma; p%re = d_Re; p%dt = d_dt
+   p%drag = d_drag; p%vdrag = d_vdrag; p%hypo = d_HYPO; p%theta = d_theta
+   call mwf_check(mwf_ic_gaussian(p, seed, scl, k0, vel_c%Re, vel_c%Im), 'mwf_ic_gaussian')
+
+   open(newunit=io_KE, status='replace', file='vel_energy.dat')
+   do while (tim_step < nsteps)
+      call vel_imposesym()
+      if (modulo(tim_step, i_save_rate2) == 0) then
+         call vel_mpt2phys(vel_c, vp)
+         call vel_energy(vp, E)
+         write(io_KE, '(2e20.12)') tim_t, E
+      end if
+      call vel_TS()
+      tim_t = tim_t + d_dt
+      tim_step = tim_step + 1
+   end do
+   close(io_KE)
+   call vel_sync_host()
+   open(newunit=uo, file='final_state.bin', access='stream', form='unformatted', status='replace')
+   write(uo) vel_c%Re, vel_c%Im
+   close(uo)
+end program mwf_drive

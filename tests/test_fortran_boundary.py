@@ -1,0 +1,71 @@
+"""The Fortran host side: the GPU variant of module velocity.
+
+CPU: it compiles against the reference's own parameters/variables/modes
+modules (when oracle/_ref has them) and against the host harness.
+GPU: the driver with main.f90's loop order runs the M4 64^2 case for 100
+steps; vel_energy.dat (printed '(2e20.12)') and the final state match the
+oracle.
+"""
+import math
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FDIR = os.path.join(ROOT, "mwf_amd", "fortran")
+DRIVER = os.path.join(FDIR, "_build", "mwf_drive")
+
+
+def _have(exe):
+    return shutil.which(exe) is not None
+
+
+@pytest.mark.skipif(not _have("amdflang"), reason="amdflang absent")
+def test_variant_compiles_against_host_harness(tmp_path):
+    for src in ("host/host_modules.f90", "mwf_capi.f90", "velocity.mwf.f90"):
+        subprocess.run(["amdflang", "-c", os.path.join(FDIR, src), "-J", str(tmp_path)], cwd=tmp_path, check=True)
+
+
+@pytest.mark.skipif(not (_have("amdflang") and os.path.exists(os.path.join(ROOT, "oracle", "_ref", "variables.mod"))),
+                    reason="reference modules not built here")
+def test_variant_compiles_against_reference_modules(tmp_path):
+    subprocess.run(["amdflang", "-c", os.path.join(FDIR, "mwf_capi.f90"), "-J", str(tmp_path)], cwd=tmp_path, check=True)
+    subprocess.run(["amdflang", "-c", os.path.join(FDIR, "velocity.mwf.f90"), "-I", os.path.join(ROOT, "oracle", "_ref"),
+                    "-J", str(tmp_path)], cwd=tmp_path, check=True)
+    assert os.path.exists(tmp_path / "velocity.mod")
+
+
+@pytest.mark.gpu
+def test_fortran_driver_matches_oracle(tmp_path):
+    import mwf_amd as mw
+    import oracle
+    assert os.path.exists(DRIVER), "build with: make -C mwf_amd/fortran"
+    nsteps = 100
+    print(f"[fortran] mwf_drive {nsteps} steps, M4 64^2 modes, seed 1", flush=True)
+    subprocess.run([DRIVER, str(nsteps), "1", "0.1", "4.0"], cwd=tmp_path, check=True, timeout=120)
+    got = np.loadtxt(tmp_path / "vel_energy.dat")
+    p = mw.make_params(4, 32, 32, 8 * math.pi, 8 * math.pi, 200.0, 0.02)
+    re, im = mw.ic_gaussian(p, 1, 0.1, 4.0)
+    o = oracle.Oracle(4, 32, 32, p.alpha, p.gamma, 200.0, 0.02)
+    o.set_state(re, im)
+    want = []
+    for s in range(nsteps):
+        want.append(o.energy())
+        o.step(1)
+    want = np.array(want)
+    assert got.shape == (nsteps, 2)
+    np.testing.assert_allclose(got[:, 0], 0.02 * np.arange(nsteps), rtol=0, atol=1e-10)
+    rel = np.abs(got[:, 1] - want) / want
+    print(f"[fortran] worst energy rel err {rel.max():.2e} (12 printed digits)")
+    assert rel.max() < 2e-12
+    fin = np.fromfile(tmp_path / "final_state.bin", dtype=np.float64)
+    K, M, NN = 7, 63, 32
+    fre, fim = fin[:K * M * NN].reshape(NN, M, K), fin[K * M * NN:].reshape(NN, M, K)
+    ore, oim = o.get_state()
+    err = math.sqrt((np.sum((fre - ore) ** 2) + np.sum((fim - oim) ** 2)) / (np.sum(ore ** 2) + np.sum(oim ** 2)))
+    print(f"[fortran] final state rel L2 {err:.2e}")
+    assert err <= 1e-12
