@@ -116,6 +116,37 @@ int mwf_profile_read(mwf_handle *h, int32_t kernel, double *total_ms, int64_t *c
  * outputs written once), for roofline accounting. */
 int mwf_kernel_bytes(mwf_handle *h, int32_t kernel, double *bytes);
 
+/* ------------------------------------------------------------------------
+ * Ensembles: many independent models stepped together.  The reference runs
+ * one model per executable (program/main.f90); an ensemble holds n members
+ * with their own parameters (Re, box, dt, basis, size, precision), groups
+ * them into size classes that step with one launch per kernel, and replays
+ * the step sequence from a hipGraph.  Member states use the mpt layout of
+ * that member's (k0, mm, nn).  Setting any member's state restarts AB2 with
+ * forward Euler for all members (nst, program/velocity.f90:24-27).
+ * ------------------------------------------------------------------------ */
+typedef struct mwf_ens mwf_ens;
+
+int mwf_ens_create(const mwf_params *members, int32_t n, mwf_ens **out);
+int mwf_ens_destroy(mwf_ens *e);
+int mwf_ens_set_state(mwf_ens *e, int32_t member, const double *re, const double *im);
+int mwf_ens_get_state(mwf_ens *e, int32_t member, double *re, double *im);
+/* nsteps x vel_TS for every member (hipGraph replay after the first step). */
+int mwf_ens_step(mwf_ens *e, int32_t nsteps);
+/* vel_energy of every member (n values). */
+int mwf_ens_energy(mwf_ens *e, double *energies);
+int mwf_ens_sync(mwf_ens *e);
+/* Record every member's energy every `every` steps (0 = off) and the first
+ * step at which it is <= threshold (relaminarisation; the reference's
+ * io_write_energy kill, program/io.f90:462-465, uses an absolute 1e-10). */
+int mwf_ens_track(mwf_ens *e, int32_t every, double threshold);
+/* Member's recorded energies (up to cap), their count, and the
+ * relaminarisation step (-1 if none). */
+int mwf_ens_trace(mwf_ens *e, int32_t member, double *energies, int32_t cap, int32_t *count,
+                  int64_t *relam_step);
+/* Number of size classes the members were grouped into. */
+int mwf_ens_classes(mwf_ens *e, int32_t *nclasses);
+
 #ifdef __cplusplus
 }
 #endif

@@ -190,3 +190,87 @@ class Velocity:
         b = ctypes.c_double()
         _check(lib().mwf_kernel_bytes(self._h, kid, ctypes.byref(b)))
         return b.value
+
+
+class Ensemble:
+    """Many independent models stepped together (mwf_ens_* in include/mwf.h).
+
+    Members may differ in every parameter; members sharing (k0, mm, nn,
+    precision) form a size class stepped by one launch per kernel, and the
+    step sequence is replayed from a hipGraph."""
+
+    def __init__(self, params_list):
+        L = lib()
+        vp = ctypes.c_void_p
+        if not getattr(L, "_ens_typed", False):
+            L.mwf_ens_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int32, ctypes.POINTER(vp)]
+            L.mwf_ens_destroy.argtypes = [vp]
+            L.mwf_ens_set_state.argtypes = [vp, ctypes.c_int32, _dp, _dp]
+            L.mwf_ens_get_state.argtypes = [vp, ctypes.c_int32, _dp, _dp]
+            L.mwf_ens_step.argtypes = [vp, ctypes.c_int32]
+            L.mwf_ens_energy.argtypes = [vp, _dp]
+            L.mwf_ens_sync.argtypes = [vp]
+            L.mwf_ens_track.argtypes = [vp, ctypes.c_int32, ctypes.c_double]
+            L.mwf_ens_trace.argtypes = [vp, ctypes.c_int32, _dp, ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
+            L.mwf_ens_classes.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+            L._ens_typed = True
+        self.params = list(params_list)
+        arr = (Params * len(self.params))(*self.params)
+        h = ctypes.c_void_p()
+        _check(L.mwf_ens_create(arr, len(self.params), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mwf_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.params)
+
+    def mpt_shape(self, i):
+        d = dims(self.params[i])
+        return (self.params[i].nn, d["M"], d["K"])
+
+    def set_state(self, i, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64).reshape(self.mpt_shape(i))
+        im = np.ascontiguousarray(im, dtype=np.float64).reshape(self.mpt_shape(i))
+        _check(lib().mwf_ens_set_state(self._h, int(i), _p(re), _p(im)))
+
+    def get_state(self, i):
+        re, im = np.empty(self.mpt_shape(i)), np.empty(self.mpt_shape(i))
+        _check(lib().mwf_ens_get_state(self._h, int(i), _p(re), _p(im)))
+        return re, im
+
+    def step(self, nsteps=1):
+        _check(lib().mwf_ens_step(self._h, int(nsteps)))
+
+    def energies(self):
+        e = np.empty(len(self.params))
+        _check(lib().mwf_ens_energy(self._h, _p(e)))
+        return e
+
+    def sync(self):
+        _check(lib().mwf_ens_sync(self._h))
+
+    def track(self, every, threshold):
+        _check(lib().mwf_ens_track(self._h, int(every), float(threshold)))
+
+    def trace(self, i, cap=1 << 20):
+        cnt, rel = ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().mwf_ens_trace(self._h, int(i), None, 0, ctypes.byref(cnt), ctypes.byref(rel)))
+        buf = np.empty(max(cnt.value, 1))
+        _check(lib().mwf_ens_trace(self._h, int(i), _p(buf), cnt.value, ctypes.byref(cnt), ctypes.byref(rel)))
+        return buf[:cnt.value], rel.value
+
+    def nclasses(self):
+        n = ctypes.c_int32()
+        _check(lib().mwf_ens_classes(self._h, ctypes.byref(n)))
+        return n.value

@@ -51,6 +51,16 @@ struct Geo {
     int dragall;
 };
 
+// Member addressing of one launch: blockIdx.z selects the member; its
+// geometry and the offsets of its buffers follow from this.
+struct Batch {
+    const Geo* geo;      // [members] geometry and physical parameters
+    size_t c_stride;     // double2 elements between members' c / onl
+    size_t u_stride;     // complex elements between members' U
+    size_t nl_stride;    // complex elements between members' NL
+    int ep_stride;       // energy partials per member
+};
+
 // wall-normal derivative table ad_k1, ad_k2 (program/variables.f90:528-531)
 __device__ __forceinline__ double ak1_of(int k, double beta) { return ((k & 1) ? -1.0 : 1.0) * k * beta; }
 __device__ __forceinline__ double ak2_of(int k, double beta) { return -k * k * beta * beta; }

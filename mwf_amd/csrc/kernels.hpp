@@ -81,11 +81,15 @@ __device__ __forceinline__ void mpt2spec_k(const Geo& g, int k, int mi, int n, d
 // of the spec fields without the laminar profile, program/velocity.f90:115-162).
 // ---------------------------------------------------------------------------
 template <typename R, int LOGMM>
-__global__ __launch_bounds__(XT) void k_xinv(Geo g, const double2* __restrict__ c, cplx_t<R>* __restrict__ U,
+__global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict__ c, cplx_t<R>* __restrict__ U,
                                              double* __restrict__ epart, const cplx_t<R>* __restrict__ twMM,
                                              const double2* __restrict__ twP, int addlam)
 {
     using CT = cplx_t<R>;
+    const Geo g = bt.geo[blockIdx.z];
+    c += blockIdx.z * bt.c_stride;
+    if (U) U += blockIdx.z * bt.u_stride;
+    if (epart) epart += blockIdx.z * bt.ep_stride;
     constexpr int MM = 1 << LOGMM;
     constexpr int LP = lpad(MM);
     constexpr int IPT = XTILE / XT; // items per thread
@@ -245,10 +249,13 @@ template <typename R, int K0, int LOGNN> constexpr size_t z_lds_bytes()
 }
 
 template <typename R, int K0, int LOGNN>
-__global__ __launch_bounds__(ZT) void k_z(Geo g, const cplx_t<R>* __restrict__ U, cplx_t<R>* __restrict__ NL,
+__global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict__ U, cplx_t<R>* __restrict__ NL,
                                           const cplx_t<R>* __restrict__ twNN, const cplx_t<R>* __restrict__ twQ)
 {
     using CT = cplx_t<R>;
+    const Geo g = bt.geo[blockIdx.z];
+    U += blockIdx.z * bt.u_stride;
+    NL += blockIdx.z * bt.nl_stride;
     using G = Gal<K0>;
     constexpr int KK = 3 * K0 - 1;
     constexpr int NN = 1 << LOGNN;
@@ -425,10 +432,11 @@ __global__ __launch_bounds__(ZT) void k_z(Geo g, const cplx_t<R>* __restrict__ U
 
 // z inverse only, to the reference phys layout (vel_mpt2phys).
 template <typename R, int K0, int LOGNN>
-__global__ __launch_bounds__(ZT) void k_zphys(Geo g, const cplx_t<R>* __restrict__ U, double* __restrict__ phys,
+__global__ __launch_bounds__(ZT) void k_zphys(Batch bt, const cplx_t<R>* __restrict__ U, double* __restrict__ phys,
                                               const cplx_t<R>* __restrict__ twNN, const cplx_t<R>* __restrict__ twQ)
 {
     using CT = cplx_t<R>;
+    const Geo g = bt.geo[0];
     constexpr int KK = 3 * K0 - 1;
     constexpr int NN = 1 << LOGNN;
     constexpr int XB = ZT / NN;
@@ -555,11 +563,15 @@ __device__ __forceinline__ double2 ab2_update(double2 cv, double2 nl, double2 ov
 // mode 1: write the nonlinear term into onl only (vel_nonlinear).
 // ---------------------------------------------------------------------------
 template <typename R, int LOGMM>
-__global__ __launch_bounds__(XT) void k_xfwd(Geo g, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
+__global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
                                              double2* __restrict__ onl, const cplx_t<R>* __restrict__ twMM,
                                              const double2* __restrict__ twP, int mode, int nst)
 {
     using CT = cplx_t<R>;
+    const Geo g = bt.geo[blockIdx.z];
+    NL += blockIdx.z * bt.nl_stride;
+    c += blockIdx.z * bt.c_stride;
+    onl += blockIdx.z * bt.c_stride;
     constexpr int MM = 1 << LOGMM;
     constexpr int LP = lpad(MM);
     constexpr int IPT = XTILE / XT;
@@ -677,10 +689,13 @@ __global__ __launch_bounds__(XT) void k_xfwd(Geo g, const cplx_t<R>* __restrict_
     }
 }
 
-// Sum of the energy partials in a fixed order with compensation.
-__global__ void k_ereduce(const double* __restrict__ part, int n, double* __restrict__ out)
+// Sum of each member's energy partials in a fixed order with compensation;
+// one thread per member.
+__global__ void k_ereduce(const double* __restrict__ part, int n, int nmem, double* __restrict__ out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int mbr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mbr < nmem) {
+        part += (size_t)mbr * n;
         double s = 0.0, comp = 0.0;
         for (int i = 0; i < n; i++) {
             const double y = part[i] - comp;
@@ -688,6 +703,6 @@ __global__ void k_ereduce(const double* __restrict__ part, int n, double* __rest
             comp = (t - s) - y;
             s = t;
         }
-        out[0] = s;
+        out[mbr] = s;
     }
 }

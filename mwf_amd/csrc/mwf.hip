@@ -1,13 +1,22 @@
-// mwf.hip -- C-ABI of libmwf.so: handle, device buffers, plans and launches.
+// mwf.hip -- C-ABI of libmwf.so: handles, device buffers, plans and launches.
 //
-// One handle is one model instance (the module state of program/velocity.f90
-// :15-17 plus the plans of program/transform.fftw5.f90:33-59).  All device
-// work goes to the handle's own stream at default priority.
+// A size class is a set of members that share (K0, MM, NN, precision): their
+// states sit back to back in device memory and every kernel steps all of
+// them in one launch, blockIdx.z selecting the member and its parameters.
+// A single model (mwf_handle, the module state of program/velocity.f90:15-17
+// plus the plans of program/transform.fftw5.f90:33-59) is one class with one
+// member; an ensemble (mwf_ens) is any number of classes whose step sequence
+// is captured once into a hipGraph and replayed.  All device work goes to the
+// handle's own stream at default priority.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/mwf.h"
@@ -21,10 +30,16 @@ static int fail(int code, const std::string& msg)
     return code;
 }
 
-#define HIPCHK(x)                                                                              \
-    do {                                                                                       \
-        hipError_t e_ = (x);                                                                   \
-        if (e_ != hipSuccess) return fail(MWF_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+#define HIPCHK(x)                                                                                          \
+    do {                                                                                                   \
+        hipError_t e_ = (x);                                                                               \
+        if (e_ != hipSuccess) return fail(MWF_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define RCCHK(x)                 \
+    do {                         \
+        int rc_ = (x);           \
+        if (rc_) return rc_;     \
     } while (0)
 
 extern "C" const char* mwf_last_error(void) { return g_err.c_str(); }
@@ -32,12 +47,14 @@ extern "C" const char* mwf_last_error(void) { return g_err.c_str(); }
 // ---------------------------------------------------------------------------
 // launchers and dispatch
 // ---------------------------------------------------------------------------
-using xinv_fn = hipError_t (*)(const Geo&, hipStream_t, const double2*, void*, double*, const void*,
-                               const double2*, int);
-using xfwd_fn = hipError_t (*)(const Geo&, hipStream_t, const void*, double2*, double2*, const void*,
-                               const double2*, int, int);
-using z_fn = hipError_t (*)(const Geo&, hipStream_t, const void*, void*, const void*, const void*);
-using zphys_fn = hipError_t (*)(const Geo&, hipStream_t, const void*, double*, const void*, const void*);
+struct Plan {
+    hipError_t (*xinv)(const Batch&, int, const Geo&, hipStream_t, const double2*, void*, double*, const void*,
+                       const double2*, int) = nullptr;
+    hipError_t (*xfwd)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, const void*,
+                       const double2*, int, int) = nullptr;
+    hipError_t (*zf)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*, const void*) = nullptr;
+    hipError_t (*zphys)(const Batch&, const Geo&, hipStream_t, const void*, double*, const void*, const void*) = nullptr;
+};
 
 template <typename F> static void set_lds(F* f, size_t bytes)
 {
@@ -45,139 +62,115 @@ template <typename F> static void set_lds(F* f, size_t bytes)
                               (int)bytes);
 }
 
+template <typename R, int LOGMM> static size_t x_lds() { return 3 * (size_t)(XTILE >> LOGMM) * lpad(1 << LOGMM) * sizeof(cplx_t<R>); }
+template <typename R, int LOGNN> static size_t zphys_lds()
+{
+    return (size_t)(ZT >> LOGNN) * ZPB * 3 * lpad(1 << LOGNN) * sizeof(cplx_t<R>);
+}
+
 template <typename R, int LOGMM>
-static hipError_t launch_xinv(const Geo& g, hipStream_t s, const double2* c, void* U, double* ep,
-                              const void* twMM, const double2* twP, int addlam)
+static hipError_t launch_xinv(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const double2* c, void* U,
+                              double* ep, const void* twMM, const double2* twP, int addlam)
 {
     using CT = cplx_t<R>;
-    const size_t lds = 3 * (size_t)(XTILE >> LOGMM) * lpad(1 << LOGMM) * sizeof(CT);
-    set_lds(k_xinv<R, LOGMM>, lds);
-    dim3 grid(g.NN / g.NTx, g.K0);
-    hipLaunchKernelGGL((k_xinv<R, LOGMM>), grid, dim3(XT), lds, s, g, c, (CT*)U, ep, (const CT*)twMM, twP,
-                       addlam);
+    dim3 grid(g.NN / g.NTx, g.K0, nmem);
+    hipLaunchKernelGGL((k_xinv<R, LOGMM>), grid, dim3(XT), (x_lds<R, LOGMM>()), s, bt, c, (CT*)U, ep, (const CT*)twMM,
+                       twP, addlam);
     return hipGetLastError();
 }
 
 template <typename R, int LOGMM>
-static hipError_t launch_xfwd(const Geo& g, hipStream_t s, const void* NL, double2* c, double2* onl,
-                              const void* twMM, const double2* twP, int mode, int nst)
+static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
+                              double2* onl, const void* twMM, const double2* twP, int mode, int nst)
 {
     using CT = cplx_t<R>;
-    const size_t lds = 3 * (size_t)(XTILE >> LOGMM) * lpad(1 << LOGMM) * sizeof(CT);
-    set_lds(k_xfwd<R, LOGMM>, lds);
-    dim3 grid(g.NN / g.NTx, g.K0);
-    hipLaunchKernelGGL((k_xfwd<R, LOGMM>), grid, dim3(XT), lds, s, g, (const CT*)NL, c, onl, (const CT*)twMM,
-                       twP, mode, nst);
+    dim3 grid(g.NN / g.NTx, g.K0, nmem);
+    hipLaunchKernelGGL((k_xfwd<R, LOGMM>), grid, dim3(XT), (x_lds<R, LOGMM>()), s, bt, (const CT*)NL, c, onl,
+                       (const CT*)twMM, twP, mode, nst);
     return hipGetLastError();
 }
 
 template <typename R, int K0, int LOGNN>
-static hipError_t launch_z(const Geo& g, hipStream_t s, const void* U, void* NL, const void* twNN,
-                           const void* twQ)
+static hipError_t launch_z(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
+                           const void* twNN, const void* twQ)
 {
     using CT = cplx_t<R>;
     constexpr int XB = ZT >> LOGNN;
-    const size_t lds = z_lds_bytes<R, K0, LOGNN>();
-    set_lds(k_z<R, K0, LOGNN>, lds);
-    dim3 grid((g.P + XB - 1) / XB);
-    hipLaunchKernelGGL((k_z<R, K0, LOGNN>), grid, dim3(ZT), lds, s, g, (const CT*)U, (CT*)NL,
-                       (const CT*)twNN, (const CT*)twQ);
+    dim3 grid((g.P + XB - 1) / XB, 1, nmem);
+    hipLaunchKernelGGL((k_z<R, K0, LOGNN>), grid, dim3(ZT), (z_lds_bytes<R, K0, LOGNN>()), s, bt, (const CT*)U,
+                       (CT*)NL, (const CT*)twNN, (const CT*)twQ);
     return hipGetLastError();
 }
 
 template <typename R, int K0, int LOGNN>
-static hipError_t launch_zphys(const Geo& g, hipStream_t s, const void* U, double* phys, const void* twNN,
-                               const void* twQ)
+static hipError_t launch_zphys(const Batch& bt, const Geo& g, hipStream_t s, const void* U, double* phys,
+                               const void* twNN, const void* twQ)
 {
     using CT = cplx_t<R>;
     constexpr int XB = ZT >> LOGNN;
-    const size_t lds = (size_t)XB * ZPB * 3 * lpad(1 << LOGNN) * sizeof(CT);
-    set_lds(k_zphys<R, K0, LOGNN>, lds);
     dim3 grid((g.P + XB - 1) / XB);
-    hipLaunchKernelGGL((k_zphys<R, K0, LOGNN>), grid, dim3(ZT), lds, s, g, (const CT*)U, phys,
+    hipLaunchKernelGGL((k_zphys<R, K0, LOGNN>), grid, dim3(ZT), (zphys_lds<R, LOGNN>()), s, bt, (const CT*)U, phys,
                        (const CT*)twNN, (const CT*)twQ);
     return hipGetLastError();
 }
 
-#define LOG_CASES(M, ...) \
-    M(4, __VA_ARGS__) M(5, __VA_ARGS__) M(6, __VA_ARGS__) M(7, __VA_ARGS__) M(8, __VA_ARGS__) M(9, __VA_ARGS__) M(10, __VA_ARGS__)
-#define LOGZ_CASES(M, ...) \
-    M(4, __VA_ARGS__) M(5, __VA_ARGS__) M(6, __VA_ARGS__) M(7, __VA_ARGS__) M(8, __VA_ARGS__) M(9, __VA_ARGS__)
-
-template <typename R> static xinv_fn pick_xinv(int lg)
+// x kernels depend on MM only, z kernels on (K0, NN) only: dispatch each
+// separately so the instantiation count stays additive.
+template <typename R> static void pick_x(int lm, Plan& p)
 {
-#define C_(L, R_) \
-    case L: return launch_xinv<R_, L>;
-    switch (lg) { LOG_CASES(C_, R) }
+    switch (lm) {
+#define C_(L)                                   \
+    case L:                                     \
+        p.xinv = launch_xinv<R, L>;             \
+        p.xfwd = launch_xfwd<R, L>;             \
+        break;
+        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10)
 #undef C_
-    return nullptr;
+    default: break;
+    }
 }
 
-template <typename R> static xfwd_fn pick_xfwd(int lg)
+template <typename R, int K0> static void pick_z(int ln, Plan& p)
 {
-#define C_(L, R_) \
-    case L: return launch_xfwd<R_, L>;
-    switch (lg) { LOG_CASES(C_, R) }
+    switch (ln) {
+#define C_(L)                                   \
+    case L:                                     \
+        p.zf = launch_z<R, K0, L>;              \
+        p.zphys = launch_zphys<R, K0, L>;       \
+        break;
+        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
-    return nullptr;
+    default: break;
+    }
 }
 
-template <typename R, int K0> static z_fn pick_z(int lg)
+template <typename R> static void prepare_x(int lm)
 {
-#define C_(L, R_, K_) \
-    case L: return launch_z<R_, K_, L>;
-    switch (lg) { LOGZ_CASES(C_, R, K0) }
+    switch (lm) {
+#define C_(L)                                         \
+    case L:                                           \
+        set_lds(k_xinv<R, L>, x_lds<R, L>());        \
+        set_lds(k_xfwd<R, L>, x_lds<R, L>());        \
+        break;
+        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10)
 #undef C_
-    return nullptr;
+    default: break;
+    }
 }
 
-template <typename R, int K0> static zphys_fn pick_zphys(int lg)
+template <typename R, int K0> static void prepare_z(int ln)
 {
-#define C_(L, R_, K_) \
-    case L: return launch_zphys<R_, K_, L>;
-    switch (lg) { LOGZ_CASES(C_, R, K0) }
+    switch (ln) {
+#define C_(L)                                                     \
+    case L:                                                       \
+        set_lds(k_z<R, K0, L>, z_lds_bytes<R, K0, L>());          \
+        set_lds(k_zphys<R, K0, L>, zphys_lds<R, L>());            \
+        break;
+        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
-    return nullptr;
+    default: break;
+    }
 }
-
-// ---------------------------------------------------------------------------
-// handle
-// ---------------------------------------------------------------------------
-enum { KID_XINV = 0, KID_Z = 1, KID_XFWD = 2, KID_ENERGY = 3, NKID = 4 };
-
-struct mwf_handle {
-    mwf_params p{};
-    Geo g{};
-    hipStream_t st = nullptr;
-    double2* c = nullptr;   // state vel_c
-    double2* onl = nullptr; // AB2 history vel_onl
-    double2* cs = nullptr;  // scratch mpt (nonlinear output, mpt2phys input)
-    void* U = nullptr;
-    void* NL = nullptr;
-    double* phys = nullptr;
-    size_t phys_elems = 0;
-    double2 *twMMd = nullptr, *twNNd = nullptr, *twPd = nullptr, *twQd = nullptr;
-    float2 *twMMf = nullptr, *twNNf = nullptr, *twQf = nullptr;
-    double *epart = nullptr, *edev = nullptr;
-    int nepart = 0;
-    double2* hstage = nullptr; // pinned host staging, one device mpt
-    size_t mpt_elems = 0;      // NN * K * ldM
-    int nst = 1;
-    xinv_fn xinv = nullptr;
-    xfwd_fn xfwd = nullptr;
-    z_fn zf = nullptr;
-    zphys_fn zphys = nullptr;
-    // profiling
-    struct Ev {
-        int id;
-        hipEvent_t a, b;
-    };
-    int prof = 0;
-    std::vector<Ev> pending;
-    std::vector<hipEvent_t> pool;
-    double ms[NKID] = {0, 0, 0, 0};
-    int64_t cnt[NKID] = {0, 0, 0, 0};
-};
 
 static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 static int ilog2(int v)
@@ -187,100 +180,43 @@ static int ilog2(int v)
     return l;
 }
 
-template <typename CT> static std::vector<CT> twiddles(int n)
+static Plan plan_for(int prec, int k0, int mm, int nn)
 {
-    std::vector<CT> t(n);
-    const long double twopi = 6.283185307179586476925286766559L;
-    for (int j = 0; j < n; j++) {
-        long double a = twopi * (long double)j / (long double)n;
-        t[j].x = (decltype(t[j].x))cosl(a);
-        t[j].y = (decltype(t[j].y))sinl(a);
+    Plan p;
+    const int lm = ilog2(mm), ln = ilog2(nn);
+    if (prec == MWF_MIXED) {
+        pick_x<float>(lm, p);
+        if (k0 == 4) pick_z<float, 4>(ln, p);
+        else pick_z<float, 6>(ln, p);
+        prepare_x<float>(lm);
+        if (k0 == 4) prepare_z<float, 4>(ln);
+        else prepare_z<float, 6>(ln);
+    } else {
+        pick_x<double>(lm, p);
+        if (k0 == 4) pick_z<double, 4>(ln, p);
+        else pick_z<double, 6>(ln, p);
+        prepare_x<double>(lm);
+        if (k0 == 4) prepare_z<double, 4>(ln);
+        else prepare_z<double, 6>(ln);
     }
-    return t;
+    return p;
 }
 
-template <typename T> static int dev_upload(T** d, const std::vector<T>& h)
+static int check_params(const mwf_params* p)
 {
-    HIPCHK(hipMalloc((void**)d, sizeof(T) * h.size()));
-    HIPCHK(hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
-    return 0;
-}
-
-static hipEvent_t ev_get(mwf_handle* h)
-{
-    if (!h->pool.empty()) {
-        hipEvent_t e = h->pool.back();
-        h->pool.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    (void)hipEventCreate(&e);
-    return e;
-}
-
-static int prof_drain(mwf_handle* h)
-{
-    for (auto& e : h->pending) {
-        HIPCHK(hipEventSynchronize(e.b));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
-        h->ms[e.id] += ms;
-        h->cnt[e.id] += 1;
-        h->pool.push_back(e.a);
-        h->pool.push_back(e.b);
-    }
-    h->pending.clear();
-    return 0;
-}
-
-// wraps one launch in profiling events when enabled
-template <typename F> static int timed(mwf_handle* h, int id, F&& f)
-{
-    hipEvent_t a = nullptr, b = nullptr;
-    if (h->prof) {
-        a = ev_get(h);
-        b = ev_get(h);
-        HIPCHK(hipEventRecord(a, h->st));
-    }
-    hipError_t e = f();
-    if (e != hipSuccess) return fail(MWF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-    if (h->prof) {
-        HIPCHK(hipEventRecord(b, h->st));
-        h->pending.push_back({id, a, b});
-        if (h->pending.size() > 4096) return prof_drain(h);
-    }
-    return 0;
-}
-
-static void destroy(mwf_handle* h)
-{
-    if (!h) return;
-    if (h->st) (void)hipStreamSynchronize(h->st);
-    prof_drain(h);
-    for (auto e : h->pool) (void)hipEventDestroy(e);
-    void* dptrs[] = {h->c, h->onl, h->cs, h->U, h->NL, h->phys, h->twMMd, h->twNNd, h->twPd, h->twQd,
-                     h->twMMf, h->twNNf, h->twQf, h->epart, h->edev};
-    for (void* p : dptrs)
-        if (p) (void)hipFree(p);
-    if (h->hstage) (void)hipHostFree(h->hstage);
-    if (h->st) (void)hipStreamDestroy(h->st);
-    delete h;
-}
-
-extern "C" int mwf_create(const mwf_params* p, mwf_handle** out)
-{
-    if (!p || !out) return fail(MWF_E_ARG, "null argument");
-    *out = nullptr;
+    if (!p) return fail(MWF_E_ARG, "null parameters");
     if (p->k0 != 4 && p->k0 != 6) return fail(MWF_E_ARG, "k0 must be 4 (M4) or 6 (M6)");
     if (!is_pow2(p->mm) || p->mm < 16 || p->mm > 1024) return fail(MWF_E_ARG, "mm must be a power of two in [16,1024]");
     if (!is_pow2(p->nn) || p->nn < 16 || p->nn > 512) return fail(MWF_E_ARG, "nn must be a power of two in [16,512]");
     if (p->precision != MWF_FP64 && p->precision != MWF_MIXED) return fail(MWF_E_ARG, "precision must be MWF_FP64 or MWF_MIXED");
     if (p->reserved != 0) return fail(MWF_E_ARG, "reserved must be 0");
     if (!(p->dt > 0) || !(p->re > 0)) return fail(MWF_E_ARG, "dt and re must be positive");
+    return 0;
+}
 
-    mwf_handle* h = new mwf_handle();
-    h->p = *p;
-    Geo& g = h->g;
+static Geo geo_of(const mwf_params* p)
+{
+    Geo g{};
     g.K0 = p->k0;
     g.K = 2 * p->k0 - 1;
     g.KK = 3 * p->k0 - 1;
@@ -304,56 +240,246 @@ extern "C" int mwf_create(const mwf_params* p, mwf_handle** out)
     g.beta = 3.1415926535897931 / 2.0; // d_beta, program/parameters.f90:60
     g.scale = 1.0 / (double)(g.P * g.Q);
     g.dragall = p->dragall ? 1 : 0;
+    return g;
+}
 
-    const int lm = ilog2(g.MM), ln = ilog2(g.NN);
-    const bool mixed = p->precision == MWF_MIXED;
-    if (mixed) {
-        h->xinv = pick_xinv<float>(lm);
-        h->xfwd = pick_xfwd<float>(lm);
-        h->zf = g.K0 == 4 ? pick_z<float, 4>(ln) : pick_z<float, 6>(ln);
-        h->zphys = g.K0 == 4 ? pick_zphys<float, 4>(ln) : pick_zphys<float, 6>(ln);
-    } else {
-        h->xinv = pick_xinv<double>(lm);
-        h->xfwd = pick_xfwd<double>(lm);
-        h->zf = g.K0 == 4 ? pick_z<double, 4>(ln) : pick_z<double, 6>(ln);
-        h->zphys = g.K0 == 4 ? pick_zphys<double, 4>(ln) : pick_zphys<double, 6>(ln);
+template <typename CT> static std::vector<CT> twiddles(int n)
+{
+    std::vector<CT> t(n);
+    const long double twopi = 6.283185307179586476925286766559L;
+    for (int j = 0; j < n; j++) {
+        long double a = twopi * (long double)j / (long double)n;
+        t[j].x = (decltype(t[j].x))cosl(a);
+        t[j].y = (decltype(t[j].y))sinl(a);
     }
-    if (!h->xinv || !h->xfwd || !h->zf || !h->zphys) {
-        destroy(h);
-        return fail(MWF_E_ARG, "no kernel for this size");
+    return t;
+}
+
+template <typename T> static int dev_upload(T** d, const T* h, size_t n)
+{
+    HIPCHK(hipMalloc((void**)d, sizeof(T) * n));
+    HIPCHK(hipMemcpy(*d, h, sizeof(T) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// size class: members sharing (K0, MM, NN, precision)
+// ---------------------------------------------------------------------------
+struct SizeClass {
+    int nmem = 0;
+    int prec = MWF_FP64;
+    Geo g0{};                 // shared dimensions (member 0's parameters)
+    std::vector<Geo> geos_h;  // per member
+    Geo* geos = nullptr;      // device copy
+    Batch bt{};
+    Plan plan;
+    size_t mpt_elems = 0;     // per member: NN * K * ldM
+    double2 *c = nullptr, *onl = nullptr, *cs = nullptr;
+    void *U = nullptr, *NL = nullptr;
+    double *epart = nullptr, *edev = nullptr;
+    int nepart = 0;           // per member
+    double2 *twMMd = nullptr, *twNNd = nullptr, *twPd = nullptr, *twQd = nullptr;
+    float2 *twMMf = nullptr, *twNNf = nullptr, *twQf = nullptr;
+    int nst = 1;
+
+    const void* twMM() const { return prec == MWF_MIXED ? (const void*)twMMf : (const void*)twMMd; }
+    const void* twNN() const { return prec == MWF_MIXED ? (const void*)twNNf : (const void*)twNNd; }
+    const void* twQ() const { return prec == MWF_MIXED ? (const void*)twQf : (const void*)twQd; }
+
+    ~SizeClass()
+    {
+        void* d[] = {geos, c, onl, cs, U, NL, epart, edev, twMMd, twNNd, twPd, twQd, twMMf, twNNf, twQf};
+        for (void* p : d)
+            if (p) (void)hipFree(p);
     }
 
-    int rc = 0;
-    auto chk = [&](hipError_t e, const char* what) {
-        if (e != hipSuccess && rc == 0) rc = fail(MWF_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    int init(const std::vector<mwf_params>& ps, hipStream_t st)
+    {
+        nmem = (int)ps.size();
+        prec = ps[0].precision;
+        g0 = geo_of(&ps[0]);
+        for (auto& p : ps) geos_h.push_back(geo_of(&p));
+        plan = plan_for(prec, g0.K0, g0.MM, g0.NN);
+        if (!plan.xinv || !plan.xfwd || !plan.zf || !plan.zphys) return fail(MWF_E_ARG, "no kernel for this size");
+        const size_t rsz = prec == MWF_MIXED ? sizeof(float2) : sizeof(double2);
+        mpt_elems = (size_t)g0.NN * g0.K * g0.ldM;
+        const size_t u_el = (size_t)g0.P * 2 * g0.KK * g0.NN, nl_el = (size_t)g0.P * g0.KK * g0.NN;
+        nepart = (g0.NN / g0.NTx) * g0.K0;
+        RCCHK(dev_upload(&geos, geos_h.data(), geos_h.size()));
+        HIPCHK(hipMalloc((void**)&c, sizeof(double2) * mpt_elems * nmem));
+        HIPCHK(hipMalloc((void**)&onl, sizeof(double2) * mpt_elems * nmem));
+        HIPCHK(hipMalloc((void**)&cs, sizeof(double2) * mpt_elems));
+        HIPCHK(hipMalloc(&U, rsz * u_el * nmem));
+        HIPCHK(hipMalloc(&NL, rsz * nl_el * nmem));
+        HIPCHK(hipMalloc((void**)&epart, sizeof(double) * nepart * nmem));
+        HIPCHK(hipMalloc((void**)&edev, sizeof(double) * nmem));
+        bt = Batch{geos, mpt_elems, u_el, nl_el, nepart};
+        auto td = [&](double2** d, int n) { auto t = twiddles<double2>(n); return dev_upload(d, t.data(), t.size()); };
+        auto tf = [&](float2** d, int n) { auto t = twiddles<float2>(n); return dev_upload(d, t.data(), t.size()); };
+        RCCHK(td(&twMMd, g0.MM));
+        RCCHK(td(&twNNd, g0.NN));
+        RCCHK(td(&twPd, g0.P));
+        RCCHK(td(&twQd, g0.Q));
+        RCCHK(tf(&twMMf, g0.MM));
+        RCCHK(tf(&twNNf, g0.NN));
+        RCCHK(tf(&twQf, g0.Q));
+        HIPCHK(hipMemsetAsync(c, 0, sizeof(double2) * mpt_elems * nmem, st));
+        HIPCHK(hipMemsetAsync(onl, 0, sizeof(double2) * mpt_elems * nmem, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+
+    // one step of every member: vel_TS (program/velocity.f90:21-41)
+    template <typename TIMED> int step(hipStream_t st, TIMED&& timed)
+    {
+        RCCHK(timed(0, [&] { return plan.xinv(bt, nmem, g0, st, c, U, nullptr, twMM(), twPd, 1); }));
+        RCCHK(timed(1, [&] { return plan.zf(bt, nmem, g0, st, U, NL, twNN(), twQ()); }));
+        RCCHK(timed(2, [&] { return plan.xfwd(bt, nmem, g0, st, NL, c, onl, twMM(), twPd, 0, nst); }));
+        nst = 0;
+        return 0;
+    }
+
+    // energies of all members into edev[0..nmem)
+    int energy(hipStream_t st)
+    {
+        hipError_t e = plan.xinv(bt, nmem, g0, st, c, nullptr, epart, twMM(), twPd, 0);
+        if (e != hipSuccess) return fail(MWF_E_HIP, std::string("energy launch: ") + hipGetErrorString(e));
+        hipLaunchKernelGGL(k_ereduce, dim3((nmem + 63) / 64), dim3(64), 0, st, epart, nepart, nmem, edev);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+};
+
+// reference mpt (re, im planes, (K, M, NN) column-major) <-> device rows
+static void to_dev_layout(const Geo& g, size_t elems, const double* re, const double* im, double2* d)
+{
+    std::memset(d, 0, sizeof(double2) * elems);
+    for (int n = 0; n < g.NN; n++)
+        for (int m = 0; m < g.M; m++)
+            for (int k = 0; k < g.K; k++) {
+                const size_t i = (size_t)k + (size_t)g.K * ((size_t)m + (size_t)g.M * n);
+                d[((size_t)n * g.K + k) * g.ldM + m] = double2{re[i], im[i]};
+            }
+}
+
+static void from_dev_layout(const Geo& g, const double2* d, double* re, double* im)
+{
+    for (int n = 0; n < g.NN; n++)
+        for (int m = 0; m < g.M; m++)
+            for (int k = 0; k < g.K; k++) {
+                const size_t i = (size_t)k + (size_t)g.K * ((size_t)m + (size_t)g.M * n);
+                const double2 v = d[((size_t)n * g.K + k) * g.ldM + m];
+                re[i] = v.x;
+                im[i] = v.y;
+            }
+}
+
+// ---------------------------------------------------------------------------
+// profiling
+// ---------------------------------------------------------------------------
+enum { KID_XINV = 0, KID_Z = 1, KID_XFWD = 2, KID_ENERGY = 3, NKID = 4 };
+
+struct Profiler {
+    struct Ev {
+        int id;
+        hipEvent_t a, b;
     };
-    chk(hipStreamCreate(&h->st), "hipStreamCreate");
-    h->mpt_elems = (size_t)g.NN * g.K * g.ldM;
-    const size_t csz = sizeof(double2) * h->mpt_elems;
-    const size_t rsz = mixed ? sizeof(float2) : sizeof(double2);
-    chk(hipMalloc((void**)&h->c, csz), "hipMalloc c");
-    chk(hipMalloc((void**)&h->onl, csz), "hipMalloc onl");
-    chk(hipMalloc((void**)&h->cs, csz), "hipMalloc scratch");
-    chk(hipMalloc(&h->U, rsz * (size_t)g.P * 2 * g.KK * g.NN), "hipMalloc U");
-    chk(hipMalloc(&h->NL, rsz * (size_t)g.P * g.KK * g.NN), "hipMalloc NL");
-    h->nepart = (g.NN / g.NTx) * g.K0;
-    chk(hipMalloc((void**)&h->epart, sizeof(double) * h->nepart), "hipMalloc epart");
-    chk(hipMalloc((void**)&h->edev, sizeof(double)), "hipMalloc e");
-    chk(hipHostMalloc((void**)&h->hstage, csz, hipHostMallocDefault), "hipHostMalloc");
-    if (rc) {
-        destroy(h);
-        return rc;
+    int on = 0;
+    std::vector<Ev> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[NKID] = {0, 0, 0, 0};
+    int64_t cnt[NKID] = {0, 0, 0, 0};
+
+    hipEvent_t get()
+    {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
     }
-    if ((rc = dev_upload(&h->twMMd, twiddles<double2>(g.MM))) || (rc = dev_upload(&h->twNNd, twiddles<double2>(g.NN))) ||
-        (rc = dev_upload(&h->twPd, twiddles<double2>(g.P))) || (rc = dev_upload(&h->twQd, twiddles<double2>(g.Q))) ||
-        (rc = dev_upload(&h->twMMf, twiddles<float2>(g.MM))) || (rc = dev_upload(&h->twNNf, twiddles<float2>(g.NN))) ||
-        (rc = dev_upload(&h->twQf, twiddles<float2>(g.Q)))) {
-        destroy(h);
-        return rc;
+    int drain()
+    {
+        for (auto& e : pending) {
+            HIPCHK(hipEventSynchronize(e.b));
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, e.a, e.b));
+            ms[e.id] += t;
+            cnt[e.id] += 1;
+            pool.push_back(e.a);
+            pool.push_back(e.b);
+        }
+        pending.clear();
+        return 0;
     }
-    chk(hipMemsetAsync(h->c, 0, csz, h->st), "hipMemset c");
-    chk(hipMemsetAsync(h->onl, 0, csz, h->st), "hipMemset onl");
-    chk(hipStreamSynchronize(h->st), "hipStreamSynchronize");
+    template <typename F> int timed(hipStream_t st, int id, F&& f)
+    {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (on) {
+            a = get();
+            b = get();
+            HIPCHK(hipEventRecord(a, st));
+        }
+        hipError_t e = f();
+        if (e != hipSuccess) return fail(MWF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        if (on) {
+            HIPCHK(hipEventRecord(b, st));
+            pending.push_back({id, a, b});
+            if (pending.size() > 4096) return drain();
+        }
+        return 0;
+    }
+    void release()
+    {
+        (void)drain();
+        for (auto e : pool) (void)hipEventDestroy(e);
+        pool.clear();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// single model handle
+// ---------------------------------------------------------------------------
+struct mwf_handle {
+    mwf_params p{};
+    hipStream_t st = nullptr;
+    std::unique_ptr<SizeClass> cls;
+    double* phys = nullptr;
+    size_t phys_elems = 0;
+    double2* hstage = nullptr; // pinned host staging, one device mpt
+    Profiler prof;
+};
+
+static void destroy(mwf_handle* h)
+{
+    if (!h) return;
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    h->prof.release();
+    h->cls.reset();
+    if (h->phys) (void)hipFree(h->phys);
+    if (h->hstage) (void)hipHostFree(h->hstage);
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+}
+
+extern "C" int mwf_create(const mwf_params* p, mwf_handle** out)
+{
+    if (!out) return fail(MWF_E_ARG, "null argument");
+    *out = nullptr;
+    RCCHK(check_params(p));
+    mwf_handle* h = new mwf_handle();
+    h->p = *p;
+    int rc = 0;
+    if (hipStreamCreate(&h->st) != hipSuccess) rc = fail(MWF_E_HIP, "hipStreamCreate failed");
+    if (!rc) {
+        h->cls.reset(new SizeClass());
+        rc = h->cls->init({*p}, h->st);
+    }
+    if (!rc && hipHostMalloc((void**)&h->hstage, sizeof(double2) * h->cls->mpt_elems, hipHostMallocDefault) != hipSuccess)
+        rc = fail(MWF_E_NOMEM, "hipHostMalloc failed");
     if (rc) {
         destroy(h);
         return rc;
@@ -368,114 +494,68 @@ extern "C" int mwf_destroy(mwf_handle* h)
     return 0;
 }
 
-static const void* twMM_of(const mwf_handle* h) { return h->p.precision == MWF_MIXED ? (const void*)h->twMMf : (const void*)h->twMMd; }
-static const void* twNN_of(const mwf_handle* h) { return h->p.precision == MWF_MIXED ? (const void*)h->twNNf : (const void*)h->twNNd; }
-static const void* twQ_of(const mwf_handle* h) { return h->p.precision == MWF_MIXED ? (const void*)h->twQf : (const void*)h->twQd; }
-
-// reference mpt (re, im planes, (K, M, NN) column-major) <-> device rows
-static void to_dev_layout(const mwf_handle* h, const double* re, const double* im, double2* d)
-{
-    const Geo& g = h->g;
-    std::memset(d, 0, sizeof(double2) * h->mpt_elems);
-    for (int n = 0; n < g.NN; n++)
-        for (int m = 0; m < g.M; m++)
-            for (int k = 0; k < g.K; k++) {
-                const size_t i = (size_t)k + (size_t)g.K * ((size_t)m + (size_t)g.M * n);
-                d[((size_t)n * g.K + k) * g.ldM + m] = double2{re[i], im[i]};
-            }
-}
-
-static void from_dev_layout(const mwf_handle* h, const double2* d, double* re, double* im)
-{
-    const Geo& g = h->g;
-    for (int n = 0; n < g.NN; n++)
-        for (int m = 0; m < g.M; m++)
-            for (int k = 0; k < g.K; k++) {
-                const size_t i = (size_t)k + (size_t)g.K * ((size_t)m + (size_t)g.M * n);
-                const double2 v = d[((size_t)n * g.K + k) * g.ldM + m];
-                re[i] = v.x;
-                im[i] = v.y;
-            }
-}
-
 static int upload_mpt(mwf_handle* h, const double* re, const double* im, double2* dst)
 {
-    to_dev_layout(h, re, im, h->hstage);
-    HIPCHK(hipMemcpyAsync(dst, h->hstage, sizeof(double2) * h->mpt_elems, hipMemcpyHostToDevice, h->st));
+    SizeClass& k = *h->cls;
+    to_dev_layout(k.g0, k.mpt_elems, re, im, h->hstage);
+    HIPCHK(hipMemcpyAsync(dst, h->hstage, sizeof(double2) * k.mpt_elems, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
 
 static int download_mpt(mwf_handle* h, const double2* src, double* re, double* im)
 {
-    HIPCHK(hipMemcpyAsync(h->hstage, src, sizeof(double2) * h->mpt_elems, hipMemcpyDeviceToHost, h->st));
+    SizeClass& k = *h->cls;
+    HIPCHK(hipMemcpyAsync(h->hstage, src, sizeof(double2) * k.mpt_elems, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
-    from_dev_layout(h, h->hstage, re, im);
+    from_dev_layout(k.g0, h->hstage, re, im);
     return 0;
 }
 
 extern "C" int mwf_set_state(mwf_handle* h, const double* re, const double* im)
 {
     if (!h || !re || !im) return fail(MWF_E_ARG, "null argument");
-    int rc = upload_mpt(h, re, im, h->c);
-    if (rc) return rc;
-    h->nst = 1;
+    RCCHK(upload_mpt(h, re, im, h->cls->c));
+    h->cls->nst = 1;
     return 0;
 }
 
 extern "C" int mwf_get_state(mwf_handle* h, double* re, double* im)
 {
     if (!h || !re || !im) return fail(MWF_E_ARG, "null argument");
-    return download_mpt(h, h->c, re, im);
-}
-
-static int run_nonlinear(mwf_handle* h, double2* cin, double2* cout, double2* hist, int mode)
-{
-    const Geo& g = h->g;
-    int rc;
-    if ((rc = timed(h, KID_XINV, [&] { return h->xinv(g, h->st, cin, h->U, nullptr, twMM_of(h), h->twPd, 1); })))
-        return rc;
-    if ((rc = timed(h, KID_Z, [&] { return h->zf(g, h->st, h->U, h->NL, twNN_of(h), twQ_of(h)); }))) return rc;
-    if ((rc = timed(h, KID_XFWD, [&] {
-             return h->xfwd(g, h->st, h->NL, cout, hist, twMM_of(h), h->twPd, mode, h->nst);
-         })))
-        return rc;
-    return 0;
+    return download_mpt(h, h->cls->c, re, im);
 }
 
 extern "C" int mwf_step(mwf_handle* h, int32_t nsteps)
 {
     if (!h) return fail(MWF_E_ARG, "null handle");
     if (nsteps < 0) return fail(MWF_E_ARG, "nsteps < 0");
-    for (int s = 0; s < nsteps; s++) {
-        int rc = run_nonlinear(h, h->c, h->c, h->onl, 0);
-        if (rc) return rc;
-        h->nst = 0;
-    }
+    auto timed = [&](int id, auto&& f) { return h->prof.timed(h->st, id, f); };
+    for (int s = 0; s < nsteps; s++) RCCHK(h->cls->step(h->st, timed));
     return 0;
 }
 
 extern "C" int mwf_nonlinear(mwf_handle* h, double* nlre, double* nlim)
 {
     if (!h || !nlre || !nlim) return fail(MWF_E_ARG, "null argument");
-    HIPCHK(hipMemsetAsync(h->cs, 0, sizeof(double2) * h->mpt_elems, h->st));
-    int rc = run_nonlinear(h, h->c, h->c, h->cs, 1);
-    if (rc) return rc;
-    return download_mpt(h, h->cs, nlre, nlim);
+    SizeClass& k = *h->cls;
+    HIPCHK(hipMemsetAsync(k.cs, 0, sizeof(double2) * k.mpt_elems, h->st));
+    auto launch = [&](hipError_t e) { return e == hipSuccess ? 0 : fail(MWF_E_HIP, hipGetErrorString(e)); };
+    RCCHK(launch(k.plan.xinv(k.bt, 1, k.g0, h->st, k.c, k.U, nullptr, k.twMM(), k.twPd, 1)));
+    RCCHK(launch(k.plan.zf(k.bt, 1, k.g0, h->st, k.U, k.NL, k.twNN(), k.twQ())));
+    // mode 1 writes the nonlinear term into its history argument (the scratch mpt)
+    RCCHK(launch(k.plan.xfwd(k.bt, 1, k.g0, h->st, k.NL, k.c, k.cs, k.twMM(), k.twPd, 1, 0)));
+    return download_mpt(h, k.cs, nlre, nlim);
 }
 
 extern "C" int mwf_energy(mwf_handle* h, double* e)
 {
     if (!h || !e) return fail(MWF_E_ARG, "null argument");
-    const Geo& g = h->g;
-    int rc = timed(h, KID_ENERGY, [&] {
-        hipError_t r = h->xinv(g, h->st, h->c, nullptr, h->epart, twMM_of(h), h->twPd, 0);
-        if (r != hipSuccess) return r;
-        hipLaunchKernelGGL(k_ereduce, dim3(1), dim3(64), 0, h->st, h->epart, h->nepart, h->edev);
-        return hipGetLastError();
-    });
-    if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(e, h->edev, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    RCCHK(h->prof.timed(h->st, KID_ENERGY, [&] {
+        int rc = h->cls->energy(h->st);
+        return rc ? hipErrorUnknown : hipSuccess;
+    }));
+    HIPCHK(hipMemcpyAsync(e, h->cls->edev, sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }
@@ -483,7 +563,8 @@ extern "C" int mwf_energy(mwf_handle* h, double* e)
 extern "C" int mwf_mpt2phys(mwf_handle* h, const double* re, const double* im, double* phys)
 {
     if (!h || !re || !im || !phys) return fail(MWF_E_ARG, "null argument");
-    const Geo& g = h->g;
+    SizeClass& k = *h->cls;
+    const Geo& g = k.g0;
     const size_t np = (size_t)g.KK * g.Q * g.P;
     if (h->phys_elems < np) {
         if (h->phys) HIPCHK(hipFree(h->phys));
@@ -491,11 +572,10 @@ extern "C" int mwf_mpt2phys(mwf_handle* h, const double* re, const double* im, d
         HIPCHK(hipMalloc((void**)&h->phys, sizeof(double) * np));
         h->phys_elems = np;
     }
-    int rc = upload_mpt(h, re, im, h->cs);
-    if (rc) return rc;
-    hipError_t e = h->xinv(g, h->st, h->cs, h->U, nullptr, twMM_of(h), h->twPd, 0);
+    RCCHK(upload_mpt(h, re, im, k.cs));
+    hipError_t e = k.plan.xinv(k.bt, 1, g, h->st, k.cs, k.U, nullptr, k.twMM(), k.twPd, 0);
     if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
-    e = h->zphys(g, h->st, h->U, h->phys, twNN_of(h), twQ_of(h));
+    e = k.plan.zphys(k.bt, g, h->st, k.U, h->phys, k.twNN(), k.twQ());
     if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
     HIPCHK(hipMemcpyAsync(phys, h->phys, sizeof(double) * np, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
@@ -512,12 +592,11 @@ extern "C" int mwf_sync(mwf_handle* h)
 extern "C" int mwf_profile_enable(mwf_handle* h, int32_t on)
 {
     if (!h) return fail(MWF_E_ARG, "null handle");
-    int rc = prof_drain(h);
-    if (rc) return rc;
-    h->prof = on ? 1 : 0;
+    RCCHK(h->prof.drain());
+    h->prof.on = on ? 1 : 0;
     for (int i = 0; i < NKID; i++) {
-        h->ms[i] = 0;
-        h->cnt[i] = 0;
+        h->prof.ms[i] = 0;
+        h->prof.cnt[i] = 0;
     }
     return 0;
 }
@@ -526,26 +605,275 @@ extern "C" int mwf_profile_read(mwf_handle* h, int32_t kernel, double* total_ms,
 {
     if (!h || !total_ms || !count) return fail(MWF_E_ARG, "null argument");
     if (kernel < 0 || kernel >= NKID) return fail(MWF_E_ARG, "kernel id out of range");
-    int rc = prof_drain(h);
-    if (rc) return rc;
-    *total_ms = h->ms[kernel];
-    *count = h->cnt[kernel];
+    RCCHK(h->prof.drain());
+    *total_ms = h->prof.ms[kernel];
+    *count = h->prof.cnt[kernel];
     return 0;
+}
+
+static double kernel_bytes(const SizeClass& k, int kernel)
+{
+    const Geo& g = k.g0;
+    const double s = k.prec == MWF_MIXED ? 8.0 : 16.0; // complex intermediate
+    const double state = 16.0 * g.K * g.M * g.NN;      // one complex FP64 mpt
+    const double uline = s * (double)g.P * g.NN;         // one x-physical line set
+    switch (kernel) {
+    case KID_XINV: return state + 2.0 * g.KK * uline;        // read c, write u and du/dx
+    case KID_Z: return 2.0 * g.KK * uline + g.KK * uline;     // read U, write NL
+    case KID_XFWD: return g.KK * uline + 4.0 * state;        // read NL, c, onl; write c, onl
+    case KID_ENERGY: return state;
+    default: return -1.0;
+    }
 }
 
 extern "C" int mwf_kernel_bytes(mwf_handle* h, int32_t kernel, double* bytes)
 {
     if (!h || !bytes) return fail(MWF_E_ARG, "null argument");
-    const Geo& g = h->g;
-    const double s = h->p.precision == MWF_MIXED ? 8.0 : 16.0; // complex intermediate
-    const double state = 16.0 * g.K * g.M * g.NN;             // one complex FP64 mpt
-    const double uline = s * (double)g.P * g.NN;                // one x-physical line set
-    switch (kernel) {
-    case KID_XINV: *bytes = state + 2.0 * g.KK * uline; break;           // read c, write u and du/dx
-    case KID_Z: *bytes = 2.0 * g.KK * uline + g.KK * uline; break;        // read U, write NL
-    case KID_XFWD: *bytes = g.KK * uline + 4.0 * state; break;           // read NL, c, onl; write c, onl
-    case KID_ENERGY: *bytes = state; break;
-    default: return fail(MWF_E_ARG, "kernel id out of range");
+    const double b = kernel_bytes(*h->cls, kernel);
+    if (b < 0) return fail(MWF_E_ARG, "kernel id out of range");
+    *bytes = b;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// ensemble
+// ---------------------------------------------------------------------------
+struct mwf_ens {
+    hipStream_t st = nullptr;
+    std::vector<std::unique_ptr<SizeClass>> classes;
+    std::vector<std::pair<int, int>> where; // member -> (class, index in class)
+    std::vector<mwf_params> params;
+    double2* hstage = nullptr;
+    size_t hstage_elems = 0;
+    double* ebuf = nullptr; // host, one energy per member
+    // step graphs by length (steps after the first), built on demand
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    int nst_all = 1;
+    std::vector<std::vector<double>> eclass; // host copy of each class's energies
+    // on-device energy tracking
+    int track_every = 0;
+    double track_thr = 0.0;
+    int64_t step_count = 0;
+    std::vector<std::vector<double>> trace; // [member][sample]
+    std::vector<int64_t> relam;             // first step with E <= threshold, or -1
+};
+
+static void ens_destroy(mwf_ens* e)
+{
+    if (!e) return;
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    for (auto& kv : e->graphs) {
+        (void)hipGraphExecDestroy(kv.second.second);
+        (void)hipGraphDestroy(kv.second.first);
     }
+    e->graphs.clear();
+    e->classes.clear();
+    if (e->hstage) (void)hipHostFree(e->hstage);
+    delete[] e->ebuf;
+    if (e->st) (void)hipStreamDestroy(e->st);
+    delete e;
+}
+
+extern "C" int mwf_ens_create(const mwf_params* members, int32_t n, mwf_ens** out)
+{
+    if (!members || !out || n <= 0) return fail(MWF_E_ARG, "bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) RCCHK(check_params(&members[i]));
+    mwf_ens* e = new mwf_ens();
+    e->params.assign(members, members + n);
+    e->where.resize(n);
+    e->ebuf = new double[n];
+    e->relam.assign(n, -1);
+    e->trace.resize(n);
+    int rc = 0;
+    if (hipStreamCreate(&e->st) != hipSuccess) rc = fail(MWF_E_HIP, "hipStreamCreate failed");
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
+    for (int i = 0; i < n; i++)
+        groups[std::make_tuple(members[i].precision, members[i].k0, members[i].mm, members[i].nn)].push_back(i);
+    size_t maxel = 0;
+    for (auto& kv : groups) {
+        if (rc) break;
+        std::vector<mwf_params> ps;
+        for (int i : kv.second) ps.push_back(members[i]);
+        std::unique_ptr<SizeClass> k(new SizeClass());
+        rc = k->init(ps, e->st);
+        if (rc) break;
+        for (size_t j = 0; j < kv.second.size(); j++) e->where[kv.second[j]] = {(int)e->classes.size(), (int)j};
+        maxel = std::max(maxel, k->mpt_elems);
+        e->classes.push_back(std::move(k));
+    }
+    if (!rc && hipHostMalloc((void**)&e->hstage, sizeof(double2) * maxel, hipHostMallocDefault) != hipSuccess)
+        rc = fail(MWF_E_NOMEM, "hipHostMalloc failed");
+    e->hstage_elems = maxel;
+    if (rc) {
+        ens_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return 0;
+}
+
+extern "C" int mwf_ens_destroy(mwf_ens* e)
+{
+    ens_destroy(e);
+    return 0;
+}
+
+extern "C" int mwf_ens_set_state(mwf_ens* e, int32_t member, const double* re, const double* im)
+{
+    if (!e || !re || !im || member < 0 || member >= (int)e->where.size()) return fail(MWF_E_ARG, "bad arguments");
+    SizeClass& k = *e->classes[e->where[member].first];
+    to_dev_layout(k.g0, k.mpt_elems, re, im, e->hstage);
+    HIPCHK(hipMemcpyAsync(k.c + k.mpt_elems * e->where[member].second, e->hstage, sizeof(double2) * k.mpt_elems,
+                          hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    // AB2 restarts with forward Euler for the whole ensemble (vel_precompute's nst)
+    for (auto& c : e->classes) c->nst = 1;
+    e->nst_all = 1;
+    return 0;
+}
+
+extern "C" int mwf_ens_get_state(mwf_ens* e, int32_t member, double* re, double* im)
+{
+    if (!e || !re || !im || member < 0 || member >= (int)e->where.size()) return fail(MWF_E_ARG, "bad arguments");
+    SizeClass& k = *e->classes[e->where[member].first];
+    HIPCHK(hipMemcpyAsync(e->hstage, k.c + k.mpt_elems * e->where[member].second, sizeof(double2) * k.mpt_elems,
+                          hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    from_dev_layout(k.g0, e->hstage, re, im);
+    return 0;
+}
+
+static int ens_energies(mwf_ens* e)
+{
+    e->eclass.resize(e->classes.size());
+    for (size_t c = 0; c < e->classes.size(); c++) {
+        SizeClass& k = *e->classes[c];
+        RCCHK(k.energy(e->st));
+        e->eclass[c].resize(k.nmem);
+        HIPCHK(hipMemcpyAsync(e->eclass[c].data(), k.edev, sizeof(double) * k.nmem, hipMemcpyDeviceToHost, e->st));
+    }
+    HIPCHK(hipStreamSynchronize(e->st));
+    for (size_t i = 0; i < e->where.size(); i++) e->ebuf[i] = e->eclass[e->where[i].first][e->where[i].second];
+    return 0;
+}
+
+extern "C" int mwf_ens_energy(mwf_ens* e, double* energies)
+{
+    if (!e || !energies) return fail(MWF_E_ARG, "null argument");
+    RCCHK(ens_energies(e));
+    std::memcpy(energies, e->ebuf, sizeof(double) * e->where.size());
+    return 0;
+}
+
+static int ens_eager_step(mwf_ens* e)
+{
+    auto plain = [&](int, auto&& f) {
+        hipError_t r = f();
+        return r == hipSuccess ? 0 : fail(MWF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(r));
+    };
+    for (auto& k : e->classes) RCCHK(k->step(e->st, plain));
+    return 0;
+}
+
+static int ens_graph(mwf_ens* e, int steps, hipGraphExec_t* exec)
+{
+    auto it = e->graphs.find(steps);
+    if (it != e->graphs.end()) {
+        *exec = it->second.second;
+        return 0;
+    }
+    HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int s = 0; s < steps && !rc; s++) rc = ens_eager_step(e);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess) return fail(MWF_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    hipGraphExec_t x = nullptr;
+    hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(MWF_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    e->graphs[steps] = {g, x};
+    *exec = x;
+    return 0;
+}
+
+static int ens_track(mwf_ens* e)
+{
+    RCCHK(ens_energies(e));
+    for (size_t i = 0; i < e->where.size(); i++) {
+        e->trace[i].push_back(e->ebuf[i]);
+        if (e->relam[i] < 0 && e->ebuf[i] <= e->track_thr) e->relam[i] = e->step_count;
+    }
+    return 0;
+}
+
+// nsteps of every member; steps after the first are replayed from a graph of
+// GRAPH_STEPS steps (energies recorded between replays when tracking is on).
+extern "C" int mwf_ens_step(mwf_ens* e, int32_t nsteps)
+{
+    if (!e) return fail(MWF_E_ARG, "null handle");
+    if (nsteps < 0) return fail(MWF_E_ARG, "nsteps < 0");
+    int done = 0;
+    if (e->track_every > 0 && e->step_count == 0 && e->trace[0].empty()) RCCHK(ens_track(e));
+    while (done < nsteps) {
+        if (e->nst_all) { // first step: forward Euler, outside the graph
+            RCCHK(ens_eager_step(e));
+            e->nst_all = 0;
+            done++;
+            e->step_count++;
+        } else {
+            int chunk = e->track_every > 0 ? e->track_every - (int)(e->step_count % e->track_every) : 10;
+            chunk = std::min(chunk, nsteps - done);
+            hipGraphExec_t x = nullptr;
+            RCCHK(ens_graph(e, chunk, &x));
+            HIPCHK(hipGraphLaunch(x, e->st));
+            done += chunk;
+            e->step_count += chunk;
+        }
+        if (e->track_every > 0 && e->step_count % e->track_every == 0) RCCHK(ens_track(e));
+    }
+    return 0;
+}
+
+extern "C" int mwf_ens_sync(mwf_ens* e)
+{
+    if (!e) return fail(MWF_E_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(e->st));
+    return 0;
+}
+
+extern "C" int mwf_ens_track(mwf_ens* e, int32_t every, double threshold)
+{
+    if (!e || every < 0) return fail(MWF_E_ARG, "bad arguments");
+    e->track_every = every;
+    e->track_thr = threshold;
+    for (auto& t : e->trace) t.clear();
+    std::fill(e->relam.begin(), e->relam.end(), (int64_t)-1);
+    return 0;
+}
+
+extern "C" int mwf_ens_trace(mwf_ens* e, int32_t member, double* energies, int32_t cap, int32_t* count,
+                             int64_t* relam_step)
+{
+    if (!e || !count || !relam_step || member < 0 || member >= (int)e->where.size()) return fail(MWF_E_ARG, "bad arguments");
+    const auto& t = e->trace[member];
+    const int n = (int)std::min<size_t>(t.size(), cap > 0 ? (size_t)cap : 0);
+    if (energies && n > 0) std::memcpy(energies, t.data(), sizeof(double) * n);
+    *count = (int32_t)t.size();
+    *relam_step = e->relam[member];
+    return 0;
+}
+
+extern "C" int mwf_ens_classes(mwf_ens* e, int32_t* nclasses)
+{
+    if (!e || !nclasses) return fail(MWF_E_ARG, "null argument");
+    *nclasses = (int32_t)e->classes.size();
     return 0;
 }
