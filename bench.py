@@ -60,6 +60,96 @@ def run_gpu(mw, precision, steps, warmup, ic):
                 energy=e)
 
 
+def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20):
+    """256 M4 members, 128x128 modes, Lx=Lz=16pi, Re ~ U[150,220], rms ~ U[0.05,0.4],
+    seeds 1000..1255, dt 0.02, FP64: batched member-steps/s against the same
+    members stepped one after another through the single-model path."""
+    rng = np.random.default_rng(1000)
+    res = rng.uniform(150.0, 220.0, 256)
+    scls = rng.uniform(0.05, 0.4, 256)
+    ps = [mw.make_params(4, 64, 64, 16 * np.pi, 16 * np.pi, float(res[i]), 0.02) for i in range(256)]
+    ens = mw.Ensemble(ps)
+    ics = [mw.ic_gaussian(ps[i], 1000 + i, float(scls[i]), 8.0) for i in range(256)]
+    for i in range(256):
+        ens.set_state(i, *ics[i])
+    ens.step(11)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(nsteps)
+    ens.sync()
+    t1 = time.perf_counter()
+    batched = 256 * nsteps / (t1 - t0)
+    e = ens.energies()
+    ens.close()
+    # sequential on the GPU: one model per member, one after another
+    vs = []
+    for i in range(seq_members):
+        v = mw.Velocity(ps[i])
+        v.set_state(*ics[i])
+        v.vel_TS(2)
+        vs.append(v)
+    vs[-1].sync()
+    t0 = time.perf_counter()
+    for v in vs:
+        v.vel_TS(seq_steps)
+    for v in vs:
+        v.sync()
+    t1 = time.perf_counter()
+    seq = seq_members * seq_steps / (t1 - t0)
+    for v in vs:
+        v.close()
+    return {"workload": "256 x M4 128x128 modes (MM=NN=64), Lx=Lz=16pi, Re U[150,220], rms U[0.05,0.4], "
+                        "seeds 1000..1255, dt 0.02, FP64",
+            "member_steps_per_s": batched, "steps": nsteps,
+            "sequential_member_steps_per_s": seq, "sequential_sample": f"{seq_members} members x {seq_steps} steps",
+            "speedup_vs_sequential": batched / seq, "finite": bool(np.all(np.isfinite(e))),
+            "graph": "hipGraph replay of 10-step chunks"}
+
+
+def ensemble_cfg5(mw, nsteps, subset=32):
+    """512 members, M4:M6 = 3:1, sizes {64,128,256,512}^2 modes, box 0.4 pi per
+    mode, Re ~ U[150,300], seeds 5000..5511, dt 0.01, mixed form; energy error
+    of a 32-member FP64 subset after the same steps."""
+    rng = np.random.default_rng(5000)
+    sizes = rng.choice([64, 128, 256, 512], 512)
+    k0s = np.where(rng.uniform(size=512) < 0.25, 6, 4)
+    res = rng.uniform(150.0, 300.0, 512)
+    ps, ps64, ics = [], [], []
+    for i in range(512):
+        nmodes = int(sizes[i])
+        L = 0.4 * np.pi * nmodes
+        kw = dict(k0=int(k0s[i]), mm=nmodes // 2, nn=nmodes // 2, lx=L, lz=L, re=float(res[i]), dt=0.01)
+        ps.append(mw.make_params(precision=mw.MWF_MIXED, **kw))
+        ps64.append(mw.make_params(precision=mw.MWF_FP64, **kw))
+        ics.append(mw.ic_gaussian(ps[i], 5000 + i, 0.2, 8.0))
+    ens = mw.Ensemble(ps)
+    for i in range(512):
+        ens.set_state(i, *ics[i])
+    ens.step(1)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(nsteps)
+    ens.sync()
+    t1 = time.perf_counter()
+    e_mixed = ens.energies()
+    ncls = ens.nclasses()
+    ens.close()
+    sub = list(range(subset))
+    e64 = mw.Ensemble([ps64[i] for i in sub])
+    for j, i in enumerate(sub):
+        e64.set_state(j, *ics[i])
+    e64.step(1 + nsteps)
+    ref = e64.energies()
+    e64.close()
+    err = np.abs(e_mixed[sub] - ref) / np.abs(ref)
+    return {"workload": "512 members, M4:M6 = 3:1, sizes {64,128,256,512}^2 modes, L = 0.4pi per mode, "
+                        "Re U[150,300], seeds 5000..5511, dt 0.01, mixed FP32/FP64",
+            "member_steps_per_s": 512 * nsteps / (t1 - t0), "steps": nsteps, "size_classes": ncls,
+            "fp64_subset_energy_rel_err_max": float(err.max()),
+            "fp64_subset_energy_rel_err_median": float(np.median(err)),
+            "finite": bool(np.all(np.isfinite(e_mixed)))}
+
+
 def cpu_baseline(ic, nsteps):
     import oracle
     threads = min(os.cpu_count() or 1, 16)
@@ -87,6 +177,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-steps", type=int, default=50)
+    ap.add_argument("--cfg5-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -135,6 +228,9 @@ def main():
                          "kernels_ms": {n: round(k["avg_ms"], 5) for n, k in mixed["kernels"].items()},
                          "energy_rel_err_vs_fp64": abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"]),
                          "finite": mixed["finite"]}
+    if not args.no_ensemble:
+        line["ensemble"] = {"m4_128_x256": ensemble_cfg4(mw, args.ens_steps),
+                            "uneven_x512_mixed": ensemble_cfg5(mw, args.cfg5_steps)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ic, args.cpu_steps)
     print(json.dumps(line), flush=True)
