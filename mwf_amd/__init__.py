@@ -32,8 +32,9 @@ LIBPATH = os.path.join(_HERE, "libmwf.so")
 MWF_FP64 = 0
 MWF_MIXED = 1
 
-KID_XINV, KID_Z, KID_XFWD, KID_ENERGY = 0, 1, 2, 3
-KERNEL_NAMES = {KID_XINV: "k_xinv", KID_Z: "k_z", KID_XFWD: "k_xfwd", KID_ENERGY: "energy"}
+KID_XINV, KID_Z, KID_XFWD, KID_ENERGY, KID_XSTEP = 0, 1, 2, 3, 4
+KERNEL_NAMES = {KID_XINV: "k_xinv", KID_Z: "k_z", KID_XFWD: "k_xfwd", KID_ENERGY: "energy",
+                KID_XSTEP: "k_xstep"}
 
 
 class MWFError(RuntimeError):

@@ -183,3 +183,63 @@ __device__ __forceinline__ void ip_fft(CT* buf, int S, const CT* __restrict__ tw
         ip_fft<CT, LOG, SIGN, T, SMAX, Ns * R>(buf, S, tw);
     }
 }
+
+// ---------------------------------------------------------------------------
+// Group FFT: one sequence of length L held by a group of G = min(64, L/8)
+// lanes of one wave, lane gl holding elements gl + G*q, q < L/G, in v[] on
+// entry and on exit (natural order).  Passes exchange data through the
+// group's own LDS region (L values, swizzled), so no block barrier is
+// needed: the lanes of a wave issue their LDS operations in program order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int L> constexpr int group_lanes() { return L / 8 < 64 ? L / 8 : 64; }
+
+template <typename CT, int LOG, int SIGN, int Ns = 1, int PASS = 0>
+__device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, const CT* __restrict__ tw)
+{
+    constexpr int L = 1 << LOG;
+    constexpr int G = group_lanes<L>();
+    if constexpr (Ns < L) {
+        constexpr int R = radix_at<L, Ns>();
+        constexpr int NBL = L / (R * G); // butterflies per lane
+        constexpr bool LAST = Ns * R == L;
+        CT x[NBL][R];
+#pragma unroll
+        for (int i = 0; i < NBL; i++) {
+            const int j = gl + G * i;
+#pragma unroll
+            for (int r = 0; r < R; r++) x[i][r] = PASS == 0 ? v[i + r * NBL] : reg[pidx(j + r * (L / R))];
+        }
+        if constexpr (PASS > 0) wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < NBL; i++) {
+            const int j = gl + G * i;
+            if constexpr (Ns > 1) twiddle<R, SIGN>(x[i], tw, (j & (Ns - 1)) * (L / (Ns * R)));
+            dftR<R, SIGN>(x[i]);
+        }
+        if constexpr (LAST) {
+            // element j + r*Ns = gl + G*(i + r*NBL)
+#pragma unroll
+            for (int i = 0; i < NBL; i++)
+#pragma unroll
+                for (int r = 0; r < R; r++) v[i + r * NBL] = x[i][r];
+        } else {
+#pragma unroll
+            for (int i = 0; i < NBL; i++) {
+                const int j = gl + G * i;
+                const int k = j & (Ns - 1);
+                const int o = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; r++) reg[pidx(o + r * Ns)] = x[i][r];
+            }
+            wave_lds_sync();
+            group_fft<CT, LOG, SIGN, Ns * R, PASS + 1>(v, reg, gl, tw);
+        }
+    }
+}

@@ -706,3 +706,222 @@ __global__ void k_ereduce(const double* __restrict__ part, int n, int nmem, doub
         out[mbr] = s;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Fused x kernel of one step: forward x transform of NL, spec2mpt, AB2 and
+// implicit update of c and onl, then the inverse x transform of the updated
+// state for the next step's z kernel (mpt2spec + laminar + d/dx), so c is read
+// once per step.  Block (n-tile, y-mode k), XS = 768 threads: each group of
+// MM/8 lanes owns one of the 3*NT sequences (t, b), 8 points per lane, and
+// transforms it with group_fft (no block barrier inside a transform); items
+// (t, r), up to 3 per thread, hold psi_k / phi_k at m = r and m = r - MM.
+// ---------------------------------------------------------------------------
+constexpr int XS = 768;
+
+template <typename R, int LOGMM>
+__global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
+                                              double2* __restrict__ onl, cplx_t<R>* __restrict__ U,
+                                              const cplx_t<R>* __restrict__ twMM, const double2* __restrict__ twP,
+                                              int nst)
+{
+    using CT = cplx_t<R>;
+    constexpr int MM = 1 << LOGMM;
+    constexpr int G = group_lanes<MM>();
+    constexpr int PL = MM / G;
+    static_assert(PL == 8, "k_xstep covers MM <= 512");
+    constexpr int IPT = (XTILE + XS - 1) / XS; // items per thread
+    constexpr int LPT = 3 * XTILE / XS;        // rows per thread in the cooperative load/store
+    extern __shared__ unsigned char smem_raw[];
+    CT* reg = reinterpret_cast<CT*>(smem_raw); // [XS/G sequences][MM], swizzled
+
+    const Geo g = bt.geo[blockIdx.z];
+    NL += blockIdx.z * bt.nl_stride;
+    c += blockIdx.z * bt.c_stride;
+    onl += blockIdx.z * bt.c_stride;
+    U += blockIdx.z * bt.u_stride;
+
+    const int NT = g.NTx;
+    const int n0 = blockIdx.x * NT;
+    const int k = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int nitems = NT * MM;
+    const int nrows = 3 * nitems; // (x, t) pairs of one line
+    const int s = tid / G, gl = tid % G;
+    const bool seq_on = s < 3 * NT;
+    CT* myreg = reg + s * MM;
+    const double ak1 = ak1_of(k, g.beta);
+    const double ak2 = ak2_of(k, g.beta);
+
+    // psi[] holds u_k until w_k arrives, phi[] holds v_k (am2+an2); after the
+    // update both hold the new state
+    double2 psi[IPT][2], phi[IPT][2];
+
+    // ---------------- forward x transform + spec2mpt ----------------
+#pragma unroll 1
+    for (int comp = 0; comp < 3; comp++) {
+        if (k == 0 && comp == 1) continue;
+        const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
+        // opaque copy of the thread index: keeps item arithmetic inside the loop
+        int tidv = tid;
+        asm volatile("" : "+v"(tidv));
+#pragma unroll
+        for (int q = 0; q < LPT; q++) {
+            const int i = tidv + XS * q;
+            if (i >= nrows) continue;
+            const int x = i / NT, t = i - x * NT;
+            const int a = x / 3, b = x - 3 * a;
+            reg[(t * 3 + b) * MM + pidx(a)] = NL[((size_t)x * g.KK + slot) * g.NN + n0 + t];
+        }
+        __syncthreads();
+        if (seq_on) {
+            CT v[PL];
+#pragma unroll
+            for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
+            group_fft<CT, LOGMM, -1>(v, myreg, gl, twMM);
+#pragma unroll
+            for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const int it = tidv + XS * q;
+            if (it >= nitems) continue;
+            const int t = it >> LOGMM, r = it & (MM - 1);
+            const int n = n0 + t;
+            const CT* bt3 = reg + t * 3 * MM;
+            const CT F0 = bt3[pidx(r)];
+            double2 Xp = double2{(double)F0.x, (double)F0.y}, Xn = Xp;
+#pragma unroll
+            for (int b = 1; b < 3; b++) {
+                const CT Fb = bt3[b * MM + pidx(r)];
+                const double fr = (double)Fb.x, fi = (double)Fb.y;
+                const double2 w1 = twP[r * b];        // m = r:      times conj(w1)
+                const double2 w2 = twP[(MM - r) * b]; // m = r - MM: times w2
+                Xp.x += fr * w1.x + fi * w1.y;
+                Xp.y += fi * w1.x - fr * w1.y;
+                Xn.x += fr * w2.x - fi * w2.y;
+                Xn.y += fi * w2.x + fr * w2.y;
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (h == 1 && r == 0) continue;
+                const double2 X = h == 0 ? Xp : Xn;
+                const int mi = h == 0 ? r : MM - 1 + r;
+                double am1, am2, an1, an2;
+                xwave(g, mi, am1, am2);
+                zwave(g, n, an1, an2);
+                if (comp == 0) {
+                    psi[q][h] = X;
+                } else if (comp == 1) {
+                    phi[q][h] = s2m_vterm(X, am2, an2);
+                } else {
+                    const double2 u = psi[q][h], vh = phi[q][h];
+                    s2m_finish(k, mi, n, u, X, vh, am1, an1, ak1, psi[q][h], phi[q][h]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---------------- update (program/velocity.f90:24-40) ----------------
+    const double dt = g.dt;
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+        const int it = tid + XS * q;
+        if (it >= nitems) continue;
+        const int t = it >> LOGMM, r = it & (MM - 1);
+        const int n = n0 + t;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if (h == 1 && r == 0) continue;
+            const int mi = h == 0 ? r : MM - 1 + r;
+            double am1, am2, an1, an2;
+            xwave(g, mi, am1, am2);
+            zwave(g, n, an1, an2);
+            const bool mean = (mi == 0 && n == 0);
+#pragma unroll
+            for (int which = 0; which < 2; which++) {
+                if (which == 1 && k == 0) continue;
+                const int slot = which == 0 ? k : g.K0 - 1 + k;
+                const size_t idx = ((size_t)n * g.K + slot) * g.ldM + mi;
+                const double2 nl = which == 0 ? psi[q][h] : phi[q][h];
+                double lhs, rhs;
+                lhsrhs_k(g, k, which, mean, am2, an2, ak2, lhs, rhs);
+                const double2 cv = c[idx];
+                const double2 ov = nst ? nl : onl[idx];
+                double2 cn = ab2_update(cv, nl, ov, lhs, rhs, dt);
+                if (mean && which == 0 && k == 0) cn.x = 0.0;
+                c[idx] = cn;
+                onl[idx] = nl;
+            }
+        }
+    }
+
+    // ---------------- inverse x transform of the new state ----------------
+#pragma unroll 1
+    for (int li = 0; li < 6; li++) {
+        const int comp = li % 3, deriv = li / 3;
+        if (k == 0 && comp == 1) continue;
+        const int line = comp * (g.K0 - 1) + k + deriv * g.KK; // spec slot of u_k, v_k, w_k
+        int tidv = tid;
+        asm volatile("" : "+v"(tidv));
+#pragma unroll
+        for (int q = 0; q < IPT; q++) {
+            const int it = tidv + XS * q;
+            if (it >= nitems) continue;
+            const int t = it >> LOGMM, r = it & (MM - 1);
+            const int n = n0 + t;
+            const double2* cn = c + (size_t)n * g.K * g.ldM;
+            double2 X[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (h == 1 && r == 0) {
+                    X[h] = double2{0.0, 0.0};
+                    continue;
+                }
+                const int mi = h == 0 ? r : MM - 1 + r;
+                // the state this thread has just written (same addresses)
+                const double2 ps = cn[(size_t)k * g.ldM + mi];
+                const double2 ph = k > 0 ? cn[(size_t)(g.K0 - 1 + k) * g.ldM + mi] : double2{0.0, 0.0};
+                double2 u, v, w;
+                mpt2spec_k(g, k, mi, n, ps, ph, true, u, v, w);
+                double2 val = comp == 0 ? u : (comp == 1 ? v : w);
+                if (deriv) { // i alpha m (program/variables.f90:606-607)
+                    double am1, am2;
+                    xwave(g, mi, am1, am2);
+                    val = double2{-(val.y * am1), val.x * am1};
+                }
+                X[h] = val;
+            }
+            CT* bt3 = reg + t * 3 * MM;
+            bt3[pidx(r)] = CT{(R)(X[0].x + X[1].x), (R)(X[0].y + X[1].y)};
+#pragma unroll
+            for (int b = 1; b < 3; b++) {
+                const double2 w1 = twP[r * b];
+                const double2 w2 = twP[(MM - r) * b]; // conj: exp(2 pi i (r-MM) b / P)
+                const double y_r = (X[0].x * w1.x - X[0].y * w1.y) + (X[1].x * w2.x + X[1].y * w2.y);
+                const double y_i = (X[0].x * w1.y + X[0].y * w1.x) + (X[1].y * w2.x - X[1].x * w2.y);
+                bt3[b * MM + pidx(r)] = CT{(R)y_r, (R)y_i};
+            }
+        }
+        __syncthreads();
+        if (seq_on) {
+            CT v[PL];
+#pragma unroll
+            for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
+            group_fft<CT, LOGMM, +1>(v, myreg, gl, twMM);
+#pragma unroll
+            for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < LPT; q++) {
+            const int i = tidv + XS * q;
+            if (i >= nrows) continue;
+            const int x = i / NT, t = i - x * NT;
+            const int a = x / 3, b = x - 3 * a;
+            U[((size_t)x * 2 * g.KK + line) * g.NN + n0 + t] = reg[(t * 3 + b) * MM + pidx(a)];
+        }
+        __syncthreads();
+    }
+}

@@ -54,6 +54,9 @@ struct Plan {
                        const double2*, int, int) = nullptr;
     hipError_t (*zf)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*, const void*) = nullptr;
     hipError_t (*zphys)(const Batch&, const Geo&, hipStream_t, const void*, double*, const void*, const void*) = nullptr;
+    // fused forward + update + next inverse (MM <= 512); null otherwise
+    hipError_t (*xstep)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
+                        const void*, const double2*, int) = nullptr;
 };
 
 template <typename F> static void set_lds(F* f, size_t bytes)
@@ -90,6 +93,19 @@ static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream
     return hipGetLastError();
 }
 
+template <typename R, int LOGMM> static size_t xstep_lds() { return (size_t)(XS / group_lanes<(1 << LOGMM)>()) * (1 << LOGMM) * sizeof(cplx_t<R>); }
+
+template <typename R, int LOGMM>
+static hipError_t launch_xstep(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
+                               double2* onl, void* U, const void* twMM, const double2* twP, int nst)
+{
+    using CT = cplx_t<R>;
+    dim3 grid(g.NN / g.NTx, g.K0, nmem);
+    hipLaunchKernelGGL((k_xstep<R, LOGMM>), grid, dim3(XS), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL, c, onl,
+                       (CT*)U, (const CT*)twMM, twP, nst);
+    return hipGetLastError();
+}
+
 template <typename R, int K0, int LOGNN>
 static hipError_t launch_z(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
                            const void* twNN, const void* twQ)
@@ -123,9 +139,14 @@ template <typename R> static void pick_x(int lm, Plan& p)
     case L:                                     \
         p.xinv = launch_xinv<R, L>;             \
         p.xfwd = launch_xfwd<R, L>;             \
+        p.xstep = launch_xstep<R, L>;           \
         break;
-        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10)
+        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
+    case 10:
+        p.xinv = launch_xinv<R, 10>;
+        p.xfwd = launch_xfwd<R, 10>;
+        break;
     default: break;
     }
 }
@@ -151,9 +172,14 @@ template <typename R> static void prepare_x(int lm)
     case L:                                           \
         set_lds(k_xinv<R, L>, x_lds<R, L>());        \
         set_lds(k_xfwd<R, L>, x_lds<R, L>());        \
+        set_lds(k_xstep<R, L>, xstep_lds<R, L>());   \
         break;
-        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9) C_(10)
+        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
+    case 10:
+        set_lds(k_xinv<R, 10>, x_lds<R, 10>());
+        set_lds(k_xfwd<R, 10>, x_lds<R, 10>());
+        break;
     default: break;
     }
 }
@@ -281,6 +307,7 @@ struct SizeClass {
     double2 *twMMd = nullptr, *twNNd = nullptr, *twPd = nullptr, *twQd = nullptr;
     float2 *twMMf = nullptr, *twNNf = nullptr, *twQf = nullptr;
     int nst = 1;
+    bool u_valid = false; // U holds the x-inverse of the current c (with the laminar profile)
 
     const void* twMM() const { return prec == MWF_MIXED ? (const void*)twMMf : (const void*)twMMd; }
     const void* twNN() const { return prec == MWF_MIXED ? (const void*)twNNf : (const void*)twNNd; }
@@ -329,12 +356,20 @@ struct SizeClass {
         return 0;
     }
 
-    // one step of every member: vel_TS (program/velocity.f90:21-41)
+    // one step of every member: vel_TS (program/velocity.f90:21-41).  With the
+    // fused x kernel a step is k_z + k_xstep, k_xstep leaving U ready for the
+    // next step; k_xinv runs only when U is stale (first step, new state).
     template <typename TIMED> int step(hipStream_t st, TIMED&& timed)
     {
-        RCCHK(timed(0, [&] { return plan.xinv(bt, nmem, g0, st, c, U, nullptr, twMM(), twPd, 1); }));
+        if (!plan.xstep || !u_valid)
+            RCCHK(timed(0, [&] { return plan.xinv(bt, nmem, g0, st, c, U, nullptr, twMM(), twPd, 1); }));
         RCCHK(timed(1, [&] { return plan.zf(bt, nmem, g0, st, U, NL, twNN(), twQ()); }));
-        RCCHK(timed(2, [&] { return plan.xfwd(bt, nmem, g0, st, NL, c, onl, twMM(), twPd, 0, nst); }));
+        if (plan.xstep) {
+            RCCHK(timed(4, [&] { return plan.xstep(bt, nmem, g0, st, NL, c, onl, U, twMM(), twPd, nst); }));
+            u_valid = true;
+        } else {
+            RCCHK(timed(2, [&] { return plan.xfwd(bt, nmem, g0, st, NL, c, onl, twMM(), twPd, 0, nst); }));
+        }
         nst = 0;
         return 0;
     }
@@ -377,7 +412,7 @@ static void from_dev_layout(const Geo& g, const double2* d, double* re, double* 
 // ---------------------------------------------------------------------------
 // profiling
 // ---------------------------------------------------------------------------
-enum { KID_XINV = 0, KID_Z = 1, KID_XFWD = 2, KID_ENERGY = 3, NKID = 4 };
+enum { KID_XINV = 0, KID_Z = 1, KID_XFWD = 2, KID_ENERGY = 3, KID_XSTEP = 4, NKID = 5 };
 
 struct Profiler {
     struct Ev {
@@ -387,8 +422,8 @@ struct Profiler {
     int on = 0;
     std::vector<Ev> pending;
     std::vector<hipEvent_t> pool;
-    double ms[NKID] = {0, 0, 0, 0};
-    int64_t cnt[NKID] = {0, 0, 0, 0};
+    double ms[NKID] = {0, 0, 0, 0, 0};
+    int64_t cnt[NKID] = {0, 0, 0, 0, 0};
 
     hipEvent_t get()
     {
@@ -517,6 +552,7 @@ extern "C" int mwf_set_state(mwf_handle* h, const double* re, const double* im)
     if (!h || !re || !im) return fail(MWF_E_ARG, "null argument");
     RCCHK(upload_mpt(h, re, im, h->cls->c));
     h->cls->nst = 1;
+    h->cls->u_valid = false;
     return 0;
 }
 
@@ -573,6 +609,7 @@ extern "C" int mwf_mpt2phys(mwf_handle* h, const double* re, const double* im, d
         h->phys_elems = np;
     }
     RCCHK(upload_mpt(h, re, im, k.cs));
+    k.u_valid = false;
     hipError_t e = k.plan.xinv(k.bt, 1, g, h->st, k.cs, k.U, nullptr, k.twMM(), k.twPd, 0);
     if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
     e = k.plan.zphys(k.bt, g, h->st, k.U, h->phys, k.twNN(), k.twQ());
@@ -622,6 +659,7 @@ static double kernel_bytes(const SizeClass& k, int kernel)
     case KID_Z: return 2.0 * g.KK * uline + g.KK * uline;     // read U, write NL
     case KID_XFWD: return g.KK * uline + 4.0 * state;        // read NL, c, onl; write c, onl
     case KID_ENERGY: return state;
+    case KID_XSTEP: return 3.0 * g.KK * uline + 4.0 * state; // read NL, c, onl; write c, onl, U
     default: return -1.0;
     }
 }
@@ -728,7 +766,10 @@ extern "C" int mwf_ens_set_state(mwf_ens* e, int32_t member, const double* re, c
                           hipMemcpyHostToDevice, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
     // AB2 restarts with forward Euler for the whole ensemble (vel_precompute's nst)
-    for (auto& c : e->classes) c->nst = 1;
+    for (auto& c : e->classes) {
+        c->nst = 1;
+        c->u_valid = false;
+    }
     e->nst_all = 1;
     return 0;
 }
