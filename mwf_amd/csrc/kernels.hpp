@@ -862,7 +862,8 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
     for (int li = 0; li < 6; li++) {
         const int comp = li % 3, deriv = li / 3;
         if (k == 0 && comp == 1) continue;
-        const int line = comp * (g.K0 - 1) + k + deriv * g.KK; // spec slot of u_k, v_k, w_k
+        // spec slot of u_k, v_k, w_k: k, K0-1+k, 2K0-1+k
+        const int line = (comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k)) + deriv * g.KK;
         int tidv = tid;
         asm volatile("" : "+v"(tidv));
 #pragma unroll
