@@ -169,6 +169,33 @@ def cpu_baseline(ic, nsteps):
                        f"FFT (FFTW 3.3.8 unavailable offline), OpenMP over x/z lines")
 
 
+def calib_copy(mib):
+    """Device-to-device copy of a known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of rocprofv3 (the copy reads and writes mib MiB)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    assert hip.hipMemset(a, 1, n) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calib_copy_mib": mib}), flush=True)
+
+
+def traffic_summary():
+    """HBM bytes per k_z launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), if any."""
+    path = os.path.join(ROOT, "profiles", "k_z_m4_1024_fp64_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,7 +207,11 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--cfg5-steps", type=int, default=10)
+    ap.add_argument("--calib-copy", type=int, default=0, help="only run a MiB-sized device copy")
     args = ap.parse_args()
+    if args.calib_copy:
+        calib_copy(args.calib_copy)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
@@ -222,6 +253,10 @@ def main():
         "kernels_ms": {n: round(k["avg_ms"], 5) for n, k in kern.items()},
         "finite": fp64["finite"],
     }
+    tr = traffic_summary()
+    if tr and tr.get("kernel") == dom_name:
+        line["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+        line["roofline"]["traffic_source"] = tr["source"]
     if mixed:
         line["mixed"] = {"steps_per_s": mixed["steps_per_s"],
                          "speedup_vs_fp64": mixed["steps_per_s"] / fp64["steps_per_s"],
