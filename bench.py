@@ -170,19 +170,17 @@ def cpu_baseline(ic, nsteps):
 
 
 def calib_copy(mib):
-    """Device-to-device copy of a known size, for scaling the FETCH_SIZE /
-    WRITE_SIZE counters of rocprofv3 (the copy reads and writes mib MiB)."""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    n = ctypes.c_size_t(mib << 20)
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
-    assert hip.hipMemset(a, 1, n) == 0
+    """A plain streaming kernel of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of rocprofv3: torch's elementwise negate over mib MiB
+    of float64 reads mib MiB and writes mib MiB per launch (three launches)."""
+    import torch
+    n = (mib << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
     for _ in range(3):
-        assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    assert float(y[n - 1]) == -1.0
     print(json.dumps({"calib_copy_mib": mib}), flush=True)
 
 

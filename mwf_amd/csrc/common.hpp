@@ -11,6 +11,30 @@ template <> struct cplx_of<double> { using type = double2; };
 template <> struct cplx_of<float> { using type = float2; };
 template <typename R> using cplx_t = typename cplx_of<R>::type;
 
+// Streaming (non-temporal) vector load/store of one complex value: data
+// that is touched once per step and should not displace reused lines in L2.
+template <typename CT> __device__ __forceinline__ void st_stream(CT* p, CT v)
+{
+    using E = decltype(v.x);
+    typedef E V2 __attribute__((ext_vector_type(2)));
+    const V2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+}
+// Store of one element of an NTx-long piece: streamed when the piece spans at
+// least 64 bytes; shorter pieces are left to merge in L2 with their neighbours.
+template <typename CT> __device__ __forceinline__ void st_piece(CT* p, CT v, bool wide)
+{
+    if (wide) st_stream(p, v);
+    else *p = v;
+}
+template <typename CT> __device__ __forceinline__ CT ld_stream(const CT* p)
+{
+    using E = decltype(p->x);
+    typedef E V2 __attribute__((ext_vector_type(2)));
+    const V2 w = __builtin_nontemporal_load(reinterpret_cast<const V2*>(p));
+    return CT{w.x, w.y};
+}
+
 template <typename CT> __device__ __forceinline__ CT cadd(CT a, CT b) { return CT{a.x + b.x, a.y + b.y}; }
 template <typename CT> __device__ __forceinline__ CT csub(CT a, CT b) { return CT{a.x - b.x, a.y - b.y}; }
 template <typename CT> __device__ __forceinline__ CT cmul(CT a, CT b)
@@ -44,7 +68,8 @@ struct Geo {
     int K0, K, KK;     // y-modes, mpt slots (2K0-1), spec slots (3K0-1)
     int MM, M, ldM;    // x: modes |m| < MM, M = 2MM-1 stored, ldM = row pitch
     int NN, P, Q;      // z modes, padded grid P = 3MM (x), Q = 3NN (z)
-    int NTx;           // z-modes per x-kernel block
+    int NTx;           // z-modes per x-kernel block (power of two)
+    int lgNT;          // log2(NTx)
     int XB;            // x-rows per z-kernel block
     double alpha, gamma, re, dt, drag, vdrag, hypo, cth, sth, beta;
     double scale;      // 1/(P*Q), transform.fftw5.f90:242
@@ -60,6 +85,32 @@ struct Batch {
     size_t nl_stride;    // complex elements between members' NL
     int ep_stride;       // energy partials per member
 };
+
+// Workgroups of a launch go round-robin to the 8 XCDs (block b on XCD b % 8),
+// each with its own L2.  xcd_tile(b, n) is a bijection of [0, n) that gives
+// every XCD a contiguous run of tiles, so that tiles sharing 128-byte lines
+// (neighbouring n-tiles or x-rows) meet in one L2.
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    const int per = n >> 3;
+    if (b >= per * 8) return b;
+    return (b & 7) * per + (b >> 3);
+}
+
+// U (the x-transformed lines, 2*KK per x row) is stored U[x][line][n]: the z
+// kernel reads whole rows.  NL (the z-forward products) is stored in panels
+// NL[tile][slot][x][t], n = tile*NTx + t: the x kernels read whole panels.
+// The writers of both store NTx- or NN-long pieces; partial-line writes cost
+// no extra HBM reads, partial-line reads would.
+__device__ __forceinline__ size_t uidx(const Geo& g, int x, int line, int n)
+{
+    return ((size_t)x * 2 * g.KK + line) * g.NN + n;
+}
+__device__ __forceinline__ size_t nidx(const Geo& g, int x, int slot, int n)
+{
+    const int tile = n >> g.lgNT, t = n & (g.NTx - 1);
+    return (((size_t)tile * g.KK + slot) * g.P + x) * g.NTx + t;
+}
 
 // wall-normal derivative table ad_k1, ad_k2 (program/variables.f90:528-531)
 __device__ __forceinline__ double ak1_of(int k, double beta) { return ((k & 1) ? -1.0 : 1.0) * k * beta; }

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict
     __shared__ double red[XT];
 
     const int NT = g.NTx;
-    const int n0 = blockIdx.x * NT;
+    const int n0 = xcd_tile(blockIdx.x, gridDim.x) * NT;
     const int k = blockIdx.y;
     const int tid = threadIdx.x;
     const int nitems = NT * MM;
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict
             if (tid < s) red[tid] += red[tid + s];
             __syncthreads();
         }
-        if (tid == 0) epart[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+        if (tid == 0) epart[blockIdx.y * gridDim.x + n0 / NT] = red[0];
     }
     if (!U) return;
 
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict
             if (it >= 3 * nitems) continue;
             const int x = it / NT, t = it - x * NT;
             const int a = x / 3, b = x - 3 * a;
-            U[((size_t)x * 2 * g.KK + line) * g.NN + n0 + t] = buf[((size_t)t * 3 + b) * LP + pidx(a)];
+            U[uidx(g, x, line, n0 + t)] = buf[((size_t)t * 3 + b) * LP + pidx(a)];
         }
         __syncthreads();
     }
@@ -225,21 +225,85 @@ __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict
 // ---------------------------------------------------------------------------
 template <typename R> constexpr int zpairs() { return sizeof(R) == 8 ? 2 : 4; }
 
-template <typename R, int K0, int LOGNN>
-__device__ __forceinline__ cplx_t<R> zload_rt(const Geo& g, const cplx_t<R>* __restrict__ Urow, int gi, int n)
-{
-    using CT = cplx_t<R>;
-    constexpr int KK = 3 * K0 - 1;
-    constexpr int NN = 1 << LOGNN;
-    if (gi >= 3 * KK) return CT{(R)0, (R)0};
-    const int fam = gi / KK, s = gi - fam * KK;
-    if (fam == 1) return Urow[(KK + s) * NN + n];
-    CT v = Urow[s * NN + n];
-    if (fam == 2) { // i gamma n (program/variables.f90:608-609)
-        const R an1 = (R)(g.gamma * n);
-        v = CT{-(v.y * an1), v.x * an1};
+// Order of the 2*NP real rows that the z kernel transforms, two per complex
+// sequence (pair pr = rows 2pr, 2pr+1).  A row is a base value (kind 0, spec
+// slot s), an x-derivative (kind 1, U line KK+s), a z-derivative (kind 2,
+// i*gamma*n times base line s) or zero padding (kind 3).  The w rows come
+// first, so that every z-derivative can be folded in as it arrives (dz<s>
+// needs only w); base u/v rows are paired with their own z-derivative and
+// loaded once; the x-derivatives (which need only u) come last.
+template <int K0> struct ZOrder {
+    static_assert(K0 % 2 == 0, "w rows are paired among themselves");
+    static constexpr int KK = 3 * K0 - 1;
+    static constexpr int NW = K0 / 2;          // pairs of w rows (base, then dz)
+    static constexpr int NS = 2 * K0 - 1;      // (base, dz) pairs of u and v
+    static constexpr int NX = (KK + 1) / 2;    // pairs of x-derivatives
+    static constexpr int NP = 2 * NW + NS + NX;
+    static constexpr int W0 = 2 * K0 - 1;      // spec slot of w_0
+    __host__ __device__ static constexpr bool shared(int pr) { return pr >= NW && pr < NW + NS; }
+    __host__ __device__ static constexpr int kind(int q)
+    {
+        const int pr = q >> 1, h = q & 1;
+        if (pr < NW) return 0;
+        if (pr < NW + NS) return h == 0 ? 0 : 2;
+        if (pr < 2 * NW + NS) return 2;
+        if (pr < NP && 2 * (pr - 2 * NW - NS) + h < KK) return 1;
+        return 3;
     }
-    return v;
+    __host__ __device__ static constexpr int slot(int q)
+    {
+        const int pr = q >> 1, h = q & 1;
+        if (pr < NW) return W0 + 2 * pr + h;
+        if (pr < NW + NS) return pr - NW;
+        if (pr < 2 * NW + NS) return W0 + 2 * (pr - NW - NS) + h;
+        return 2 * (pr - 2 * NW - NS) + h;
+    }
+};
+
+// Every row appears once; each dz row after all w base rows, each dx row after
+// all u base rows (the order in which the z kernel consumes them).
+template <int K0> constexpr bool zorder_ok()
+{
+    using ZO = ZOrder<K0>;
+    constexpr int KK = ZO::KK;
+    int cnt[3][KK] = {};
+    int last_w = -1, last_u = -1, first_dz = 1 << 30, first_dx = 1 << 30;
+    for (int q = 0; q < 2 * ZO::NP; q++) {
+        const int k = ZO::kind(q), sl = ZO::slot(q);
+        if (k == 3) continue;
+        if (k < 0 || k > 3 || sl < 0 || sl >= KK) return false;
+        cnt[k][sl]++;
+        if (k == 0 && sl >= ZO::W0) last_w = q;
+        if (k == 0 && sl < K0) last_u = q;
+        if (k == 2 && q < first_dz) first_dz = q;
+        if (k == 1 && q < first_dx) first_dx = q;
+    }
+    for (int k = 0; k < 3; k++)
+        for (int sl = 0; sl < KK; sl++)
+            if (cnt[k][sl] != 1) return false;
+    for (int pr = 0; pr < ZO::NP; pr++)
+        if (ZO::shared(pr) && (ZO::kind(2 * pr) != 0 || ZO::kind(2 * pr + 1) != 2 ||
+                               ZO::slot(2 * pr) != ZO::slot(2 * pr + 1)))
+            return false;
+    return last_w < first_dz && last_u < first_dx;
+}
+static_assert(zorder_ok<4>() && zorder_ok<6>(), "z row order");
+
+// i gamma n v (program/variables.f90:608-609)
+template <typename R>
+__device__ __forceinline__ cplx_t<R> zderiv(const Geo& g, cplx_t<R> v, int n)
+{
+    const R an1 = (R)(g.gamma * n);
+    return cplx_t<R>{-(v.y * an1), v.x * an1};
+}
+
+template <typename R, int K0>
+__device__ __forceinline__ cplx_t<R> zrow(const Geo& g, const cplx_t<R>* __restrict__ U, int x, int kind, int s, int n)
+{
+    constexpr int KK = 3 * K0 - 1;
+    if (kind == 3) return cplx_t<R>{(R)0, (R)0};
+    const cplx_t<R> v = U[uidx(g, x, kind == 1 ? KK + s : s, n)];
+    return kind == 2 ? zderiv<R>(g, v, n) : v;
 }
 
 template <typename R, int K0, int LOGNN> constexpr size_t z_lds_bytes()
@@ -260,9 +324,9 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     constexpr int KK = 3 * K0 - 1;
     constexpr int NN = 1 << LOGNN;
     constexpr int XB = ZT / NN;
-    constexpr int NC = 3 * KK;                      // u, du/dx, du/dz
+    using ZO = ZOrder<K0>;
     constexpr int ZP = zpairs<R>();                 // pairs per batch
-    constexpr int NPI = (NC + 1) / 2, NBI = (NPI + ZP - 1) / ZP;
+    constexpr int NBI = (ZO::NP + ZP - 1) / ZP;
     constexpr int NPO = (KK + 1) / 2, NBO = (NPO + ZP - 1) / ZP;
     constexpr int SEQ = XB * ZP * 3;
     constexpr int LP = lpad(NN);
@@ -276,9 +340,10 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
 
     const int tid = threadIdx.x;
     const int xl = tid >> LOGNN, r = tid & (NN - 1);
-    const int x0 = blockIdx.x * XB;
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * XB;
     const int x = x0 + xl;
     const bool valid = x < g.P;
+    const bool wide = g.NTx * (int)sizeof(CT) >= 64;
 
     // fold task f of this thread: tau = tid + ZT*f -> (row fx, pair fp, index fr)
     CT pf[FT][4];
@@ -288,14 +353,20 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
             const int tau = tid + ZT * f;
             const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
             const int xg = x0 + fx;
-            const int g1 = batch * 2 * ZP + 2 * fp;
+            const int pr = batch * ZP + fp;
             const int r2 = fr == 0 ? H : NN - fr;
             if (xg < g.P) {
-                const CT* Urow = U + (size_t)xg * 2 * KK * NN;
-                pf[f][0] = zload_rt<R, K0, LOGNN>(g, Urow, g1, fr);
-                pf[f][1] = zload_rt<R, K0, LOGNN>(g, Urow, g1, r2);
-                pf[f][2] = zload_rt<R, K0, LOGNN>(g, Urow, g1 + 1, fr);
-                pf[f][3] = zload_rt<R, K0, LOGNN>(g, Urow, g1 + 1, r2);
+                const int ka = ZO::kind(2 * pr), sa = ZO::slot(2 * pr);
+                pf[f][0] = zrow<R, K0>(g, U, xg, ka, sa, fr);
+                pf[f][1] = zrow<R, K0>(g, U, xg, ka, sa, r2);
+                if (ZO::shared(pr)) {
+                    pf[f][2] = zderiv<R>(g, pf[f][0], fr);
+                    pf[f][3] = zderiv<R>(g, pf[f][1], r2);
+                } else {
+                    const int kb = ZO::kind(2 * pr + 1), sb = ZO::slot(2 * pr + 1);
+                    pf[f][2] = zrow<R, K0>(g, U, xg, kb, sb, fr);
+                    pf[f][3] = zrow<R, K0>(g, U, xg, kb, sb, r2);
+                }
             } else {
                 pf[f][0] = pf[f][1] = pf[f][2] = pf[f][3] = CT{(R)0, (R)0};
             }
@@ -363,14 +434,12 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
             for (int b = 0; b < 3; b++) {
                 const CT zv = F[((xl * ZP + p) * 3 + b) * LP + pidx(r)];
                 static_for<0, 2>([&](auto hi) {
-                    constexpr int gi = BN * 2 * ZP + 2 * p + decltype(hi)::value;
-                    if constexpr (gi < NC) {
-                        const R val = decltype(hi)::value == 0 ? zv.x : zv.y;
-                        constexpr int fam = gi / KK, s = gi % KK;
-                        if constexpr (fam == 0) vb[b][s] = val;
-                        else if constexpr (fam == 1) G::template dx<s>(vb[b], val, an[b]);
-                        else G::template dz<s>(vb[b], val, an[b]);
-                    }
+                    constexpr int q = BN * 2 * ZP + 2 * p + decltype(hi)::value;
+                    constexpr int kd = ZO::kind(q), s = ZO::slot(q);
+                    const R val = decltype(hi)::value == 0 ? zv.x : zv.y;
+                    if constexpr (kd == 0) vb[b][s] = val;
+                    else if constexpr (kd == 1) G::template dx<s>(vb[b], val, an[b]);
+                    else if constexpr (kd == 2) G::template dz<s>(vb[b], val, an[b]);
                 });
             }
         });
@@ -420,9 +489,8 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
                     }
                 }
                 if (valid) {
-                    CT* out = NL + (size_t)x * KK * NN;
-                    out[c1 * NN + r] = cscale(X1, scale);
-                    if constexpr (c2 < KK) out[c2 * NN + r] = cscale(X2, scale);
+                    st_piece(&NL[nidx(g, x, c1, r)], cscale(X1, scale), wide);
+                    if constexpr (c2 < KK) st_piece(&NL[nidx(g, x, c2, r)], cscale(X2, scale), wide);
                 }
             }
         });
@@ -449,20 +517,20 @@ __global__ __launch_bounds__(ZT) void k_zphys(Batch bt, const cplx_t<R>* __restr
     const int xl = tid >> LOGNN, r = tid & (NN - 1);
     const int x = blockIdx.x * XB + xl;
     const bool valid = x < g.P;
-    const CT* Urow = U + (size_t)(valid ? x : 0) * 2 * KK * NN;
+    const int xr = valid ? x : 0;
     const int Q = 3 * NN;
     for (int B = 0; B < NBATCH; B++) {
         for (int p = 0; p < ZPB; p++) {
             const int g1 = B * 2 * ZPB + 2 * p, g2 = g1 + 1;
             CT Z[3];
             if (valid && g1 < KK) {
-                const CT a1 = Urow[g1 * NN + r];
-                const CT a2 = g2 < KK ? Urow[g2 * NN + r] : CT{(R)0, (R)0};
+                const CT a1 = U[uidx(g, xr, g1, r)];
+                const CT a2 = g2 < KK ? U[uidx(g, xr, g2, r)] : CT{(R)0, (R)0};
                 if (r == 0) {
                     Z[0] = Z[1] = Z[2] = CT{a1.x, a2.x};
                 } else {
-                    const CT b1 = Urow[g1 * NN + NN - r];
-                    const CT b2 = g2 < KK ? Urow[g2 * NN + NN - r] : CT{(R)0, (R)0};
+                    const CT b1 = U[uidx(g, xr, g1, NN - r)];
+                    const CT b2 = g2 < KK ? U[uidx(g, xr, g2, NN - r)] : CT{(R)0, (R)0};
                     Z[0] = CT{(a1.x + b1.x) - (a2.y - b2.y), (a1.y - b1.y) + (a2.x + b2.x)};
                     for (int b = 1; b < 3; b++) {
                         const CT w1 = twQ[r * b];
@@ -579,7 +647,7 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
     CT* buf = reinterpret_cast<CT*>(smem_raw);
 
     const int NT = g.NTx;
-    const int n0 = blockIdx.x * NT;
+    const int n0 = xcd_tile(blockIdx.x, gridDim.x) * NT;
     const int k = blockIdx.y;
     const int tid = threadIdx.x;
     const int nitems = NT * MM;
@@ -598,7 +666,7 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
             if (it >= 3 * nitems) continue;
             const int x = it / NT, t = it - x * NT;
             const int a = x / 3, b = x - 3 * a;
-            buf[((size_t)t * 3 + b) * LP + pidx(a)] = NL[((size_t)x * g.KK + slot) * g.NN + n0 + t];
+            buf[((size_t)t * 3 + b) * LP + pidx(a)] = NL[nidx(g, x, slot, n0 + t)];
         }
         __syncthreads();
         ip_fft<CT, LOGMM, -1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
@@ -741,7 +809,7 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
     U += blockIdx.z * bt.u_stride;
 
     const int NT = g.NTx;
-    const int n0 = blockIdx.x * NT;
+    const int n0 = xcd_tile(blockIdx.x, gridDim.x) * NT;
     const int k = blockIdx.y;
     const int tid = threadIdx.x;
     const int nitems = NT * MM;
@@ -770,7 +838,7 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
             if (i >= nrows) continue;
             const int x = i / NT, t = i - x * NT;
             const int a = x / 3, b = x - 3 * a;
-            reg[(t * 3 + b) * MM + pidx(a)] = NL[((size_t)x * g.KK + slot) * g.NN + n0 + t];
+            reg[(t * 3 + b) * MM + pidx(a)] = ld_stream(&NL[nidx(g, x, slot, n0 + t)]);
         }
         __syncthreads();
         if (seq_on) {
@@ -852,11 +920,12 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
                 double2 cn = ab2_update(cv, nl, ov, lhs, rhs, dt);
                 if (mean && which == 0 && k == 0) cn.x = 0.0;
                 c[idx] = cn;
-                onl[idx] = nl;
+                st_stream(&onl[idx], nl);
             }
         }
     }
 
+    const bool wide = g.NTx * (int)sizeof(CT) >= 64;
     // ---------------- inverse x transform of the new state ----------------
 #pragma unroll 1
     for (int li = 0; li < 6; li++) {
@@ -921,7 +990,7 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
             if (i >= nrows) continue;
             const int x = i / NT, t = i - x * NT;
             const int a = x / 3, b = x - 3 * a;
-            U[((size_t)x * 2 * g.KK + line) * g.NN + n0 + t] = reg[(t * 3 + b) * MM + pidx(a)];
+            st_piece(&U[uidx(g, x, line, n0 + t)], reg[(t * 3 + b) * MM + pidx(a)], wide);
         }
         __syncthreads();
     }

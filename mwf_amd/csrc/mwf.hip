@@ -253,6 +253,7 @@ static Geo geo_of(const mwf_params* p)
     g.P = 3 * p->mm;
     g.Q = 3 * p->nn;
     g.NTx = std::min(p->nn, XTILE / p->mm);
+    g.lgNT = ilog2(g.NTx);
     g.XB = ZT / p->nn;
     g.alpha = p->alpha;
     g.gamma = p->gamma;
