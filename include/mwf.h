@@ -42,7 +42,7 @@ extern "C" {
 typedef struct mwf_params {
     int32_t k0;         /* i_K0: 4 (modes.M4.f90) or 6 (modes.M6.f90) */
     int32_t mm;         /* i_MM, power of two, 16..1024 */
-    int32_t nn;         /* i_NN, power of two, 16..512 */
+    int32_t nn;         /* i_NN, power of two, 16..1024 */
     int32_t dragall;    /* s_dragall */
     int32_t precision;  /* MWF_FP64 or MWF_MIXED */
     int32_t reserved;   /* must be 0 */

@@ -306,6 +306,47 @@ __device__ __forceinline__ cplx_t<R> zrow(const Geo& g, const cplx_t<R>* __restr
     return kind == 2 ? zderiv<R>(g, v, n) : v;
 }
 
+// Radix-3 fold of one phase b of the 3/2-padded z inverse (c2r of length
+// Q = 3 NN, program/transform.fftw5.f90:83): physical z = 3 r + b is the NN-point
+// inverse FFT of Y_b[s] = X[s] w^{sb} + conj(X[NN-s]) w^{-(NN-s)b}, w = exp(2 pi i/Q).
+// Two real rows share one complex sequence Z = Y(row 1) + i Y(row 2).  Fold task
+// fr < NN/2 writes Z[fr] and Z[NN-fr] from a = X[fr], m = X[NN-fr] of both rows
+// (fr = 0: the DC bin with its imaginary part dropped, and the self-mirrored
+// NN/2 bin, Y = 2 Re(X w^{(NN/2) b})).
+template <typename R>
+__device__ __forceinline__ void zfold_b(cplx_t<R>* __restrict__ seq, int b, int fr, int NN, cplx_t<R> a1,
+                                        cplx_t<R> b1, cplx_t<R> a2, cplx_t<R> b2,
+                                        const cplx_t<R>* __restrict__ twQ)
+{
+    using CT = cplx_t<R>;
+    const int H = NN / 2;
+    if (fr == 0) {
+        seq[pidx(0)] = CT{a1.x, a2.x};
+        if (b == 0) {
+            seq[pidx(H)] = CT{(R)2 * b1.x, (R)2 * b2.x};
+        } else {
+            const CT w = twQ[H * b];
+            const CT p1 = cmul(b1, w), p2 = cmul(b2, w);
+            seq[pidx(H)] = CT{(R)2 * p1.x, (R)2 * p2.x};
+        }
+        return;
+    }
+    const int fm = NN - fr;
+    CT P1a = a1, P2a = b1, P1b = a2, P2b = b2; // X[s] w^{sb}, X[NN-s] w^{(NN-s)b}
+    if (b > 0) {
+        const CT w1 = twQ[fr * b], w2 = twQ[fm * b];
+        P1a = cmul(a1, w1);
+        P2a = cmul(b1, w2);
+        P1b = cmul(a2, w1);
+        P2b = cmul(b2, w2);
+    }
+    // row 1: Y[fr] = P1a + conj(P2a), Y[fm] = P2a + conj(P1a); row 2 likewise
+    const CT y1r = CT{P1a.x + P2a.x, P1a.y - P2a.y}, y1m = CT{P2a.x + P1a.x, P2a.y - P1a.y};
+    const CT y2r = CT{P1b.x + P2b.x, P1b.y - P2b.y}, y2m = CT{P2b.x + P1b.x, P2b.y - P1b.y};
+    seq[pidx(fr)] = CT{y1r.x - y2r.y, y1r.y + y2r.x};
+    seq[pidx(fm)] = CT{y1m.x - y2m.y, y1m.y + y2m.x};
+}
+
 template <typename R, int K0, int LOGNN> constexpr size_t z_lds_bytes()
 {
     constexpr int NN = 1 << LOGNN, XB = ZT / NN, ZP = zpairs<R>();
@@ -390,38 +431,9 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
             const int tau = tid + ZT * f;
             const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
             CT* seq = A + ((fx * ZP + fp) * 3) * LP;
-            const CT a1 = pf[f][0], b1 = pf[f][1], a2 = pf[f][2], b2 = pf[f][3];
-            if (fr == 0) {
-                // s = 0: the imaginary part of the DC bin is dropped (c2r);
-                // s = NN/2 is its own mirror: Y = 2 Re(X w^{(NN/2) b})
 #pragma unroll
-                for (int b = 0; b < 3; b++) seq[b * LP + pidx(0)] = CT{a1.x, a2.x};
-                seq[pidx(H)] = CT{(R)2 * b1.x, (R)2 * b2.x};
-#pragma unroll
-                for (int b = 1; b < 3; b++) {
-                    const CT w = twQ[H * b];
-                    const CT p1 = cmul(b1, w), p2 = cmul(b2, w);
-                    seq[b * LP + pidx(H)] = CT{(R)2 * p1.x, (R)2 * p2.x};
-                }
-            } else {
-                const int fm = NN - fr;
-#pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    CT P1a = a1, P2a = b1, P1b = a2, P2b = b2; // X[s] w^{sb}, X[NN-s] w^{(NN-s)b}
-                    if (b > 0) {
-                        const CT w1 = twQ[fr * b], w2 = twQ[fm * b];
-                        P1a = cmul(a1, w1);
-                        P2a = cmul(b1, w2);
-                        P1b = cmul(a2, w1);
-                        P2b = cmul(b2, w2);
-                    }
-                    // row 1: Y[fr] = P1a + conj(P2a), Y[fm] = P2a + conj(P1a); row 2 likewise
-                    const CT y1r = CT{P1a.x + P2a.x, P1a.y - P2a.y}, y1m = CT{P2a.x + P1a.x, P2a.y - P1a.y};
-                    const CT y2r = CT{P1b.x + P2b.x, P1b.y - P2b.y}, y2m = CT{P2b.x + P1b.x, P2b.y - P1b.y};
-                    seq[b * LP + pidx(fr)] = CT{y1r.x - y2r.y, y1r.y + y2r.x};
-                    seq[b * LP + pidx(fm)] = CT{y1m.x - y2m.y, y1m.y + y2m.x};
-                }
-            }
+            for (int b = 0; b < 3; b++)
+                zfold_b<R>(seq + b * LP, b, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twQ);
         }
         if constexpr (BN + 1 < NBI) prefetch(BN + 1);
         __syncthreads();
@@ -498,16 +510,169 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     });
 }
 
-// z inverse only, to the reference phys layout (vel_mpt2phys).
+// ---------------------------------------------------------------------------
+// z pass with the three fold phases in sequence (b = 0, 1, 2), for rows whose
+// per-point state does not fit the registers of one block when all three
+// phases are live (NN = 1024, and the 17-slot M6 basis).  A block of ZB
+// threads holds XB = ZB/NN x-rows, one point (x, z = 3r + b) per thread in a
+// phase: vb[KK] base values and an[KK] product accumulators.  Each phase
+// re-reads the U row (from L2), runs the inverse transforms of its own fold,
+// the products and the forward transforms, and adds its share
+// F_b[m] w^{-mb} of the r2c output to NL; the third phase scales by 1/(PQ).
+// NL therefore holds partial sums between phases; each element is read and
+// written only by the thread that owns it, in phase order, which makes the
+// result identical to the three-phase sum of k_z.
+// ---------------------------------------------------------------------------
+constexpr int ZB = 1024;
+template <typename R> constexpr int zb_seq() { return 8; } // sequences per batch
+template <int NP, int SMAX> constexpr int zb_split() { return (NP + SMAX - 1) / SMAX; }
+template <int NP, int SMAX> constexpr int zb_per() { return (NP + zb_split<NP, SMAX>() - 1) / zb_split<NP, SMAX>(); }
+template <typename R> constexpr size_t zb_lds_bytes() { return sizeof(cplx_t<R>) * (size_t)zb_seq<R>() * ZB; }
+
 template <typename R, int K0, int LOGNN>
-__global__ __launch_bounds__(ZT) void k_zphys(Batch bt, const cplx_t<R>* __restrict__ U, double* __restrict__ phys,
+__global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict__ U, cplx_t<R>* __restrict__ NL,
+                                           const cplx_t<R>* __restrict__ twNN, const cplx_t<R>* __restrict__ twQ)
+{
+    using CT = cplx_t<R>;
+    const Geo g = bt.geo[blockIdx.z];
+    U += blockIdx.z * bt.u_stride;
+    NL += blockIdx.z * bt.nl_stride;
+    using G = Gal<K0>;
+    using ZO = ZOrder<K0>;
+    constexpr int KK = 3 * K0 - 1;
+    constexpr int NN = 1 << LOGNN;
+    static_assert(NN <= ZB, "one point per thread");
+    constexpr int XB = ZB / NN;
+    constexpr int LP = lpad(NN);
+    constexpr int H = NN / 2;
+    constexpr int SMAX = zb_seq<R>() / XB;                    // pairs per batch that fit
+    constexpr int ZP = zb_per<ZO::NP, SMAX>();                // inverse pairs per batch
+    constexpr int NBI = zb_split<ZO::NP, SMAX>();
+    constexpr int NPO = (KK + 1) / 2;
+    constexpr int ZO_ = zb_per<NPO, SMAX>();                  // forward pairs per batch
+    constexpr int NBO = zb_split<NPO, SMAX>();
+    extern __shared__ unsigned char smem_raw[];
+    CT* A = reinterpret_cast<CT*>(smem_raw);                  // [XB*ZP][LP]
+
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * XB;
+    const R scale = (R)g.scale;
+
+#pragma unroll 1
+    for (int b = 0; b < 3; b++) {
+        R vb[KK], an[KK];
+#pragma unroll
+        for (int s = 0; s < KK; s++) an[s] = (R)0;
+
+        static_for<0, NBI>([&](auto bi) {
+            constexpr int BN = decltype(bi)::value;
+            // opaque copy of the thread index: every index below is recomputed
+            // per batch instead of being hoisted (and kept live) across batches
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            const int xl = tid >> LOGNN, r = tid & (NN - 1);
+            // fold tasks tau = (row fx, pair fp, index fr)
+#pragma unroll 1
+            for (int tau = tid; tau < XB * ZP * H; tau += ZB) {
+                const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
+                const int xg = x0 + fx;
+                const int pr = BN * ZP + fp;
+                const int r2 = fr == 0 ? H : NN - fr;
+                CT a1{(R)0, (R)0}, b1{(R)0, (R)0}, a2{(R)0, (R)0}, b2{(R)0, (R)0};
+                if (xg < g.P && pr < ZO::NP) {
+                    const int ka = ZO::kind(2 * pr), sa = ZO::slot(2 * pr);
+                    a1 = zrow<R, K0>(g, U, xg, ka, sa, fr);
+                    b1 = zrow<R, K0>(g, U, xg, ka, sa, r2);
+                    if (ZO::shared(pr)) {
+                        a2 = zderiv<R>(g, a1, fr);
+                        b2 = zderiv<R>(g, b1, r2);
+                    } else {
+                        const int kb = ZO::kind(2 * pr + 1), sb = ZO::slot(2 * pr + 1);
+                        a2 = zrow<R, K0>(g, U, xg, kb, sb, fr);
+                        b2 = zrow<R, K0>(g, U, xg, kb, sb, r2);
+                    }
+                }
+                zfold_b<R>(A + (fx * ZP + fp) * LP, b, fr, NN, a1, b1, a2, b2, twQ);
+            }
+            __syncthreads();
+            ip_fft<CT, LOGNN, +1, ZB, XB * ZP>(A, XB * ZP, twNN, tid);
+            static_for<0, ZP>([&](auto pi) {
+                constexpr int p = decltype(pi)::value;
+                const CT zv = A[(xl * ZP + p) * LP + pidx(r)];
+                static_for<0, 2>([&](auto hi) {
+                    constexpr int q = BN * 2 * ZP + 2 * p + decltype(hi)::value;
+                    constexpr int kd = ZO::kind(q), sl = ZO::slot(q);
+                    const R val = decltype(hi)::value == 0 ? zv.x : zv.y;
+                    if constexpr (kd == 0) vb[sl] = val;
+                    else if constexpr (kd == 1) G::template dx<sl>(vb, val, an);
+                    else if constexpr (kd == 2) G::template dz<sl>(vb, val, an);
+                });
+            });
+            __syncthreads();
+        });
+        G::yterms(vb, an, (R)g.beta);
+
+        // forward z transform of this phase's products, two per sequence
+        static_for<0, NBO>([&](auto ri) {
+            constexpr int RD = decltype(ri)::value;
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            const int xl = tid >> LOGNN, r = tid & (NN - 1);
+            const int rm = (NN - r) & (NN - 1);
+            const int x = x0 + xl;
+            const bool valid = x < g.P;
+            static_for<0, ZO_>([&](auto pi) {
+                constexpr int p = decltype(pi)::value;
+                constexpr int c1 = 2 * (RD * ZO_ + p), c2 = c1 + 1;
+                if constexpr (c1 < KK) {
+                    R v2 = (R)0;
+                    if constexpr (c2 < KK) v2 = an[c2];
+                    A[(xl * ZO_ + p) * LP + pidx(r)] = CT{an[c1], v2};
+                }
+            });
+            __syncthreads();
+            ip_fft<CT, LOGNN, -1, ZB, XB * ZO_>(A, XB * ZO_, twNN, tid);
+            static_for<0, ZO_>([&](auto pi) {
+                constexpr int p = decltype(pi)::value;
+                constexpr int c1 = 2 * (RD * ZO_ + p), c2 = c1 + 1;
+                if constexpr (c1 < KK) {
+                    const CT zn = A[(xl * ZO_ + p) * LP + pidx(r)];
+                    const CT zm = A[(xl * ZO_ + p) * LP + pidx(rm)];
+                    CT X1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
+                    CT X2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
+                    if (b > 0) {
+                        const CT w = cconj(twQ[r * b]);
+                        X1 = cmul(X1, w);
+                        X2 = cmul(X2, w);
+                    }
+                    if (valid) {
+                        CT* o1 = &NL[nidx(g, x, c1, r)];
+                        if (b > 0) X1 = cadd(*o1, X1);
+                        *o1 = b == 2 ? cscale(X1, scale) : X1;
+                        if constexpr (c2 < KK) {
+                            CT* o2 = &NL[nidx(g, x, c2, r)];
+                            if (b > 0) X2 = cadd(*o2, X2);
+                            *o2 = b == 2 ? cscale(X2, scale) : X2;
+                        }
+                    }
+                }
+            });
+            __syncthreads();
+        });
+    }
+}
+
+// z inverse only, to the reference phys layout (vel_mpt2phys).
+template <int LOGNN> constexpr int zphys_threads() { return (1 << LOGNN) > ZT ? (1 << LOGNN) : ZT; }
+template <typename R, int K0, int LOGNN>
+__global__ __launch_bounds__(zphys_threads<LOGNN>()) void k_zphys(Batch bt, const cplx_t<R>* __restrict__ U, double* __restrict__ phys,
                                               const cplx_t<R>* __restrict__ twNN, const cplx_t<R>* __restrict__ twQ)
 {
     using CT = cplx_t<R>;
     const Geo g = bt.geo[0];
     constexpr int KK = 3 * K0 - 1;
     constexpr int NN = 1 << LOGNN;
-    constexpr int XB = ZT / NN;
+    constexpr int TZ = zphys_threads<LOGNN>();
+    constexpr int XB = TZ / NN;
     constexpr int NBATCH = (KK + 2 * ZPB - 1) / (2 * ZPB);
     constexpr int SEQ = XB * ZPB * 3;
     constexpr int LP = lpad(NN);
@@ -546,7 +711,7 @@ __global__ __launch_bounds__(ZT) void k_zphys(Batch bt, const cplx_t<R>* __restr
             for (int b = 0; b < 3; b++) buf[((xl * ZPB + p) * 3 + b) * LP + pidx(r)] = Z[b];
         }
         __syncthreads();
-        ip_fft<CT, LOGNN, +1, ZT, SEQ>(buf, SEQ, twNN);
+        ip_fft<CT, LOGNN, +1, TZ, SEQ>(buf, SEQ, twNN);
         for (int p = 0; p < ZPB; p++) {
             for (int b = 0; b < 3; b++) {
                 const CT zv = buf[((xl * ZPB + p) * 3 + b) * LP + pidx(r)];

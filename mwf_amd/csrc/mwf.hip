@@ -68,7 +68,7 @@ template <typename F> static void set_lds(F* f, size_t bytes)
 template <typename R, int LOGMM> static size_t x_lds() { return 3 * (size_t)(XTILE >> LOGMM) * lpad(1 << LOGMM) * sizeof(cplx_t<R>); }
 template <typename R, int LOGNN> static size_t zphys_lds()
 {
-    return (size_t)(ZT >> LOGNN) * ZPB * 3 * lpad(1 << LOGNN) * sizeof(cplx_t<R>);
+    return (size_t)(zphys_threads<LOGNN>() >> LOGNN) * ZPB * 3 * lpad(1 << LOGNN) * sizeof(cplx_t<R>);
 }
 
 template <typename R, int LOGMM>
@@ -119,13 +119,26 @@ static hipError_t launch_z(const Batch& bt, int nmem, const Geo& g, hipStream_t 
 }
 
 template <typename R, int K0, int LOGNN>
+static hipError_t launch_zb(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
+                            const void* twNN, const void* twQ)
+{
+    using CT = cplx_t<R>;
+    constexpr int XB = ZB >> LOGNN;
+    dim3 grid((g.P + XB - 1) / XB, 1, nmem);
+    hipLaunchKernelGGL((k_zb<R, K0, LOGNN>), grid, dim3(ZB), (zb_lds_bytes<R>()), s, bt, (const CT*)U, (CT*)NL,
+                       (const CT*)twNN, (const CT*)twQ);
+    return hipGetLastError();
+}
+
+template <typename R, int K0, int LOGNN>
 static hipError_t launch_zphys(const Batch& bt, const Geo& g, hipStream_t s, const void* U, double* phys,
                                const void* twNN, const void* twQ)
 {
     using CT = cplx_t<R>;
-    constexpr int XB = ZT >> LOGNN;
+    constexpr int TZ = zphys_threads<LOGNN>();
+    constexpr int XB = TZ >> LOGNN;
     dim3 grid((g.P + XB - 1) / XB);
-    hipLaunchKernelGGL((k_zphys<R, K0, LOGNN>), grid, dim3(ZT), (zphys_lds<R, LOGNN>()), s, bt, (const CT*)U, phys,
+    hipLaunchKernelGGL((k_zphys<R, K0, LOGNN>), grid, dim3(TZ), (zphys_lds<R, LOGNN>()), s, bt, (const CT*)U, phys,
                        (const CT*)twNN, (const CT*)twQ);
     return hipGetLastError();
 }
@@ -161,6 +174,10 @@ template <typename R, int K0> static void pick_z(int ln, Plan& p)
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
+    case 10: // one x-row per 1024-thread block: the three fold phases in sequence
+        p.zf = launch_zb<R, K0, 10>;
+        p.zphys = launch_zphys<R, K0, 10>;
+        break;
     default: break;
     }
 }
@@ -194,6 +211,10 @@ template <typename R, int K0> static void prepare_z(int ln)
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
+    case 10:
+        set_lds(k_zb<R, K0, 10>, zb_lds_bytes<R>());
+        set_lds(k_zphys<R, K0, 10>, zphys_lds<R, 10>());
+        break;
     default: break;
     }
 }
@@ -233,7 +254,7 @@ static int check_params(const mwf_params* p)
     if (!p) return fail(MWF_E_ARG, "null parameters");
     if (p->k0 != 4 && p->k0 != 6) return fail(MWF_E_ARG, "k0 must be 4 (M4) or 6 (M6)");
     if (!is_pow2(p->mm) || p->mm < 16 || p->mm > 1024) return fail(MWF_E_ARG, "mm must be a power of two in [16,1024]");
-    if (!is_pow2(p->nn) || p->nn < 16 || p->nn > 512) return fail(MWF_E_ARG, "nn must be a power of two in [16,512]");
+    if (!is_pow2(p->nn) || p->nn < 16 || p->nn > 1024) return fail(MWF_E_ARG, "nn must be a power of two in [16,1024]");
     if (p->precision != MWF_FP64 && p->precision != MWF_MIXED) return fail(MWF_E_ARG, "precision must be MWF_FP64 or MWF_MIXED");
     if (p->reserved != 0) return fail(MWF_E_ARG, "reserved must be 0");
     if (!(p->dt > 0) || !(p->re > 0)) return fail(MWF_E_ARG, "dt and re must be positive");
