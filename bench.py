@@ -33,8 +33,8 @@ WORKLOAD = dict(k0=4, mm=512, nn=512, lx=128 * np.pi, lz=128 * np.pi, re=180.0, 
 IC = dict(seed=2, scl=0.3, k0=8.0)
 
 
-def run_gpu(mw, precision, steps, warmup, ic):
-    p = mw.make_params(precision=precision, **WORKLOAD)
+def run_gpu(mw, precision, steps, warmup, ic, workload=WORKLOAD):
+    p = mw.make_params(precision=precision, **workload)
     v = mw.Velocity(p)
     v.set_state(*ic)
     v.vel_TS(warmup)
@@ -60,10 +60,13 @@ def run_gpu(mw, precision, steps, warmup, ic):
                 energy=e)
 
 
-def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20):
+def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20, track_every=50):
     """256 M4 members, 128x128 modes, Lx=Lz=16pi, Re ~ U[150,220], rms ~ U[0.05,0.4],
-    seeds 1000..1255, dt 0.02, FP64: batched member-steps/s against the same
-    members stepped one after another through the single-model path."""
+    seeds 1000..1255, dt 0.02, FP64: batched member-steps/s over nsteps (5000 by
+    default), with every member's energy traced every track_every steps and its
+    relaminarisation step (perturbation energy below 1e-6 of the laminar 0.5)
+    recorded on the device; then the same members stepped one after another
+    through the single-model path."""
     rng = np.random.default_rng(1000)
     res = rng.uniform(150.0, 220.0, 256)
     scls = rng.uniform(0.05, 0.4, 256)
@@ -72,7 +75,7 @@ def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20):
     ics = [mw.ic_gaussian(ps[i], 1000 + i, float(scls[i]), 8.0) for i in range(256)]
     for i in range(256):
         ens.set_state(i, *ics[i])
-    ens.step(11)
+    ens.track(track_every, 1e-6 * 0.5)
     ens.sync()
     t0 = time.perf_counter()
     ens.step(nsteps)
@@ -80,6 +83,11 @@ def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20):
     t1 = time.perf_counter()
     batched = 256 * nsteps / (t1 - t0)
     e = ens.energies()
+    relam = []
+    for i in range(256):
+        trace, rstep = ens.trace(i)
+        relam.append(rstep)
+    relam = np.array(relam)
     ens.close()
     # sequential on the GPU: one model per member, one after another
     vs = []
@@ -98,12 +106,68 @@ def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20):
     seq = seq_members * seq_steps / (t1 - t0)
     for v in vs:
         v.close()
+    done = relam[relam >= 0]
     return {"workload": "256 x M4 128x128 modes (MM=NN=64), Lx=Lz=16pi, Re U[150,220], rms U[0.05,0.4], "
                         "seeds 1000..1255, dt 0.02, FP64",
             "member_steps_per_s": batched, "steps": nsteps,
             "sequential_member_steps_per_s": seq, "sequential_sample": f"{seq_members} members x {seq_steps} steps",
             "speedup_vs_sequential": batched / seq, "finite": bool(np.all(np.isfinite(e))),
-            "graph": "hipGraph replay of 10-step chunks"}
+            "graph": f"hipGraph replay of {track_every}-step chunks, energy traced every {track_every} steps",
+            "relaminarised": int(done.size), "turbulent_at_end": int(256 - done.size),
+            "relam_time_median": float(np.median(done) * 0.02) if done.size else None,
+            "relam_time_min": float(done.min() * 0.02) if done.size else None,
+            "relam_time_max": float(done.max() * 0.02) if done.size else None}
+
+
+def config_m6_2048(mw, steps, err_steps, warmup=3):
+    """M6 basis, 2048x2048 modes (MM=NN=1024, 3072^2 grid), Lx=Lz=256pi, Re=250,
+    dt=0.01, seed 3 (rms 0.3, k0 8): FP64 and mixed steps/s, then both forms
+    stepped side by side from the same state with the per-step energy error
+    and the state error of the mixed run against the FP64 run."""
+    wl = dict(k0=6, mm=1024, nn=1024, lx=256 * np.pi, lz=256 * np.pi, re=250.0, dt=0.01)
+    p = mw.make_params(**wl)
+    ic = mw.ic_gaussian(p, 3, 0.3, 8.0)
+    f64 = run_gpu(mw, mw.MWF_FP64, steps, warmup, ic, wl)
+    mix = run_gpu(mw, mw.MWF_MIXED, steps, warmup, ic, wl)
+    v64 = mw.Velocity(mw.make_params(precision=mw.MWF_FP64, **wl))
+    v32 = mw.Velocity(mw.make_params(precision=mw.MWF_MIXED, **wl))
+    v64.set_state(*ic)
+    v32.set_state(*ic)
+    e_err, s_err = [], []
+    for s in range(1, err_steps + 1):
+        v64.vel_TS(1)
+        v32.vel_TS(1)
+        e64, e32 = v64.vel_energy(), v32.vel_energy()
+        e_err.append(abs(e32 - e64) / abs(e64))
+        if s % max(1, err_steps // 4) == 0:
+            a, b = v64.get_state(), v32.get_state()
+            num = np.sqrt(((a[0] - b[0]) ** 2).sum() + ((a[1] - b[1]) ** 2).sum())
+            den = np.sqrt((a[0] ** 2).sum() + (a[1] ** 2).sum())
+            s_err.append([s, float(num / den)])
+    v64.close()
+    v32.close()
+    e_err = np.array(e_err)
+    return {"workload": "M6 basis, 2048x2048 modes (MM=NN=1024, 3072^2 grid), Lx=Lz=256pi, Re=250, dt=0.01, "
+                        "seed 3 (rms 0.3, k0 8)",
+            "fp64_steps_per_s": f64["steps_per_s"], "mixed_steps_per_s": mix["steps_per_s"],
+            "speedup_mixed": mix["steps_per_s"] / f64["steps_per_s"], "steps": steps,
+            "fp64_kernels_ms": {n: round(k["avg_ms"], 4) for n, k in f64["kernels"].items()},
+            "mixed_kernels_ms": {n: round(k["avg_ms"], 4) for n, k in mix["kernels"].items()},
+            "mixed_vs_fp64": {"steps": err_steps, "energy_rel_err_per_step_max": float(e_err.max()),
+                              "energy_rel_err_at_steps": {str(k): float(e_err[k - 1]) for k in
+                                                          (1, 10, err_steps // 2, err_steps) if k <= err_steps},
+                              "state_rel_l2_at_steps": s_err},
+            "finite": f64["finite"] and mix["finite"]}
+
+
+def config_m4_64(mw, steps):
+    """M4, 64x64 modes (96^2 grid), Lx=Lz=8pi, Re=200, dt=0.02, seed 1 (rms 0.1,
+    k0 4), FP64: the reference's CPU-sized case, steps/s on the GPU."""
+    wl = dict(k0=4, mm=32, nn=32, lx=8 * np.pi, lz=8 * np.pi, re=200.0, dt=0.02)
+    ic = mw.ic_gaussian(mw.make_params(**wl), 1, 0.1, 4.0)
+    r = run_gpu(mw, mw.MWF_FP64, steps, 10, ic, wl)
+    return {"workload": "M4, 64x64 modes (96^2 grid), Lx=Lz=8pi, Re=200, dt=0.02, seed 1 (rms 0.1, k0 4), FP64",
+            "steps_per_s": r["steps_per_s"], "steps": steps, "finite": r["finite"], "energy": r["energy"]}
 
 
 def ensemble_cfg5(mw, nsteps, subset=32):
@@ -203,7 +267,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--ens-steps", type=int, default=50)
+    ap.add_argument("--ens-steps", type=int, default=5000)
+    ap.add_argument("--m6-steps", type=int, default=100)
+    ap.add_argument("--m6-err-steps", type=int, default=100)
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--cfg5-steps", type=int, default=10)
     ap.add_argument("--calib-copy", type=int, default=0, help="only run a MiB-sized device copy")
     args = ap.parse_args()
@@ -261,6 +328,9 @@ def main():
                          "kernels_ms": {n: round(k["avg_ms"], 5) for n, k in mixed["kernels"].items()},
                          "energy_rel_err_vs_fp64": abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"]),
                          "finite": mixed["finite"]}
+    if not args.no_configs:
+        line["configs"] = {"m4_64": config_m4_64(mw, 1000),
+                           "m6_2048": config_m6_2048(mw, args.m6_steps, args.m6_err_steps)}
     if not args.no_ensemble:
         line["ensemble"] = {"m4_128_x256": ensemble_cfg4(mw, args.ens_steps),
                             "uneven_x512_mixed": ensemble_cfg5(mw, args.cfg5_steps)}
