@@ -271,7 +271,7 @@ def main():
     ap.add_argument("--m6-steps", type=int, default=100)
     ap.add_argument("--m6-err-steps", type=int, default=100)
     ap.add_argument("--no-configs", action="store_true")
-    ap.add_argument("--cfg5-steps", type=int, default=10)
+    ap.add_argument("--cfg5-steps", type=int, default=200)
     ap.add_argument("--calib-copy", type=int, default=0, help="only run a MiB-sized device copy")
     args = ap.parse_args()
     if args.calib_copy:
