@@ -106,6 +106,11 @@ __device__ __forceinline__ size_t uidx(const Geo& g, int x, int line, int n)
 {
     return ((size_t)x * 2 * g.KK + line) * g.NN + n;
 }
+// NL panel (tile of n0, slot): P * NTx contiguous values, element x * NTx + t.
+template <typename CT> __device__ __forceinline__ const CT* nl_panel(const Geo& g, const CT* NL, int n0, int slot)
+{
+    return NL + ((size_t)(n0 >> g.lgNT) * g.KK + slot) * g.P * g.NTx;
+}
 __device__ __forceinline__ size_t nidx(const Geo& g, int x, int slot, int n)
 {
     const int tile = n >> g.lgNT, t = n & (g.NTx - 1);

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict
             if (tid < s) red[tid] += red[tid + s];
             __syncthreads();
         }
-        if (tid == 0) epart[blockIdx.y * gridDim.x + n0 / NT] = red[0];
+        if (tid == 0) epart[blockIdx.y * gridDim.x + (n0 >> g.lgNT)] = red[0];
     }
     if (!U) return;
 
@@ -202,13 +202,15 @@ __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict
         __syncthreads();
         ip_fft<CT, LOGMM, +1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
         // write U[x][line][n0 + t]
+        CT* Ul = U + (size_t)line * g.NN + n0;
+        const int ustride = 2 * g.KK * g.NN;
 #pragma unroll
         for (int q = 0; q < 3 * IPT; q++) {
             const int it = tid + q * XT;
             if (it >= 3 * nitems) continue;
-            const int x = it / NT, t = it - x * NT;
+            const int x = it >> g.lgNT, t = it & (NT - 1);
             const int a = x / 3, b = x - 3 * a;
-            U[uidx(g, x, line, n0 + t)] = buf[((size_t)t * 3 + b) * LP + pidx(a)];
+            Ul[x * ustride + t] = buf[((size_t)t * 3 + b) * LP + pidx(a)];
         }
         __syncthreads();
     }
@@ -824,14 +826,15 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
     for (int comp = 0; comp < 3; comp++) {
         if (k == 0 && comp == 1) continue;
         const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
+        const CT* pan = nl_panel(g, NL, n0, slot);
         // load NL rows, decimated by b = x mod 3
 #pragma unroll
         for (int q = 0; q < 3 * IPT; q++) {
             const int it = tid + q * XT;
             if (it >= 3 * nitems) continue;
-            const int x = it / NT, t = it - x * NT;
+            const int x = it >> g.lgNT, t = it & (NT - 1);
             const int a = x / 3, b = x - 3 * a;
-            buf[((size_t)t * 3 + b) * LP + pidx(a)] = NL[nidx(g, x, slot, n0 + t)];
+            buf[((size_t)t * 3 + b) * LP + pidx(a)] = pan[it];
         }
         __syncthreads();
         ip_fft<CT, LOGMM, -1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
@@ -997,13 +1000,14 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
         // opaque copy of the thread index: keeps item arithmetic inside the loop
         int tidv = tid;
         asm volatile("" : "+v"(tidv));
+        const CT* pan = nl_panel(g, NL, n0, slot);
 #pragma unroll
         for (int q = 0; q < LPT; q++) {
             const int i = tidv + XS * q;
             if (i >= nrows) continue;
-            const int x = i / NT, t = i - x * NT;
+            const int x = i >> g.lgNT, t = i & (NT - 1);
             const int a = x / 3, b = x - 3 * a;
-            reg[(t * 3 + b) * MM + pidx(a)] = ld_stream(&NL[nidx(g, x, slot, n0 + t)]);
+            reg[(t * 3 + b) * MM + pidx(a)] = ld_stream(pan + i);
         }
         __syncthreads();
         if (seq_on) {
@@ -1149,13 +1153,15 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
             for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
         }
         __syncthreads();
+        CT* Ul = U + (size_t)line * g.NN + n0;
+        const int ustride = 2 * g.KK * g.NN;
 #pragma unroll
         for (int q = 0; q < LPT; q++) {
             const int i = tidv + XS * q;
             if (i >= nrows) continue;
-            const int x = i / NT, t = i - x * NT;
+            const int x = i >> g.lgNT, t = i & (NT - 1);
             const int a = x / 3, b = x - 3 * a;
-            st_piece(&U[uidx(g, x, line, n0 + t)], reg[(t * 3 + b) * MM + pidx(a)], wide);
+            st_piece(Ul + x * ustride + t, reg[(t * 3 + b) * MM + pidx(a)], wide);
         }
         __syncthreads();
     }
