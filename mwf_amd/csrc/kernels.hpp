@@ -349,10 +349,14 @@ __device__ __forceinline__ void zfold_b(cplx_t<R>* __restrict__ seq, int b, int 
     seq[pidx(fm)] = CT{y1m.x - y2m.y, y1m.y + y2m.x};
 }
 
+// two transform buffers, then copies of the twiddle tables: NN entries of
+// exp(2 pi i j/NN) and the first 2 NN entries of exp(2 pi i j/Q) (fold and
+// r2c unpacking use indices below 2 NN).  Twiddles read from LDS instead of
+// global memory keep their latency off every transform pass.
 template <typename R, int K0, int LOGNN> constexpr size_t z_lds_bytes()
 {
     constexpr int NN = 1 << LOGNN, XB = ZT / NN, ZP = zpairs<R>();
-    return sizeof(cplx_t<R>) * 2 * (size_t)XB * ZP * 3 * lpad(NN);
+    return sizeof(cplx_t<R>) * (2 * (size_t)XB * ZP * 3 * lpad(NN) + 3 * (size_t)NN);
 }
 
 template <typename R, int K0, int LOGNN>
@@ -380,6 +384,8 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     extern __shared__ unsigned char smem_raw[];
     CT* A = reinterpret_cast<CT*>(smem_raw);        // [SEQ][LP]
     CT* B = A + SEQ * LP;
+    CT* twNl = B + SEQ * LP;                        // [NN]
+    CT* twQl = twNl + NN;                           // [2 NN]
 
     const int tid = threadIdx.x;
     const int xl = tid >> LOGNN, r = tid & (NN - 1);
@@ -424,6 +430,9 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
         for (int s = 0; s < KK; s++) an[b][s] = (R)0;
 
     prefetch(0);
+    for (int i = tid; i < NN; i += ZT) twNl[i] = twNN[i];
+    for (int i = tid; i < 2 * NN; i += ZT) twQl[i] = twQ[i];
+    __syncthreads();
     static_for<0, NBI>([&](auto bi) {
         constexpr int BN = decltype(bi)::value;
         // fold: Y_b[s] = X[s] w^{sb} + conj(X[NN-s]) w^{-(NN-s)b}, w = exp(2 pi i/Q),
@@ -435,11 +444,11 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
             CT* seq = A + ((fx * ZP + fp) * 3) * LP;
 #pragma unroll
             for (int b = 0; b < 3; b++)
-                zfold_b<R>(seq + b * LP, b, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twQ);
+                zfold_b<R>(seq + b * LP, b, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twQl);
         }
         if constexpr (BN + 1 < NBI) prefetch(BN + 1);
         __syncthreads();
-        pp_fft<CT, LOGNN, +1, ZT>(A, B, SEQ, twNN);
+        pp_fft<CT, LOGNN, +1, ZT>(A, B, SEQ, twNl);
         const CT* F = INB ? B : A;
         // consume: thread (xl, a = r)
         static_for<0, ZP>([&](auto pi) {
@@ -479,7 +488,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
             }
         });
         __syncthreads();
-        pp_fft<CT, LOGNN, -1, ZT>(A, B, SEQ, twNN);
+        pp_fft<CT, LOGNN, -1, ZT>(A, B, SEQ, twNl);
         const CT* F = INB ? B : A;
         static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
@@ -497,7 +506,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
                         X1 = F1;
                         X2 = F2;
                     } else {
-                        const CT w = cconj(twQ[r * b]);
+                        const CT w = cconj(twQl[r * b]);
                         X1 = cadd(X1, cmul(F1, w));
                         X2 = cadd(X2, cmul(F2, w));
                     }
@@ -969,6 +978,8 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
     constexpr int LPT = 3 * XTILE / XS;        // rows per thread in the cooperative load/store
     extern __shared__ unsigned char smem_raw[];
     CT* reg = reinterpret_cast<CT*>(smem_raw); // [XS/G sequences][MM], swizzled
+    CT* twMl = reg + (XS / G) * MM;            // exp(2 pi i j/MM), j < MM
+    double2* twPl = reinterpret_cast<double2*>(twMl + MM); // exp(2 pi i j/P), j <= 2 MM
 
     const Geo g = bt.geo[blockIdx.z];
     NL += blockIdx.z * bt.nl_stride;
@@ -992,6 +1003,9 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
     // update both hold the new state
     double2 psi[IPT][2], phi[IPT][2];
 
+    for (int i = tid; i < MM; i += XS) twMl[i] = twMM[i];
+    for (int i = tid; i <= 2 * MM; i += XS) twPl[i] = twP[i];
+
     // ---------------- forward x transform + spec2mpt ----------------
 #pragma unroll 1
     for (int comp = 0; comp < 3; comp++) {
@@ -1014,7 +1028,7 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
             CT v[PL];
 #pragma unroll
             for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
-            group_fft<CT, LOGMM, -1>(v, myreg, gl, twMM);
+            group_fft<CT, LOGMM, -1>(v, myreg, gl, twMl);
 #pragma unroll
             for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
         }
@@ -1032,8 +1046,8 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
             for (int b = 1; b < 3; b++) {
                 const CT Fb = bt3[b * MM + pidx(r)];
                 const double fr = (double)Fb.x, fi = (double)Fb.y;
-                const double2 w1 = twP[r * b];        // m = r:      times conj(w1)
-                const double2 w2 = twP[(MM - r) * b]; // m = r - MM: times w2
+                const double2 w1 = twPl[r * b];        // m = r:      times conj(w1)
+                const double2 w2 = twPl[(MM - r) * b]; // m = r - MM: times w2
                 Xp.x += fr * w1.x + fi * w1.y;
                 Xp.y += fi * w1.x - fr * w1.y;
                 Xn.x += fr * w2.x - fi * w2.y;
@@ -1136,8 +1150,8 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
             bt3[pidx(r)] = CT{(R)(X[0].x + X[1].x), (R)(X[0].y + X[1].y)};
 #pragma unroll
             for (int b = 1; b < 3; b++) {
-                const double2 w1 = twP[r * b];
-                const double2 w2 = twP[(MM - r) * b]; // conj: exp(2 pi i (r-MM) b / P)
+                const double2 w1 = twPl[r * b];
+                const double2 w2 = twPl[(MM - r) * b]; // conj: exp(2 pi i (r-MM) b / P)
                 const double y_r = (X[0].x * w1.x - X[0].y * w1.y) + (X[1].x * w2.x + X[1].y * w2.y);
                 const double y_i = (X[0].x * w1.y + X[0].y * w1.x) + (X[1].y * w2.x - X[1].x * w2.y);
                 bt3[b * MM + pidx(r)] = CT{(R)y_r, (R)y_i};
@@ -1148,7 +1162,7 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
             CT v[PL];
 #pragma unroll
             for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
-            group_fft<CT, LOGMM, +1>(v, myreg, gl, twMM);
+            group_fft<CT, LOGMM, +1>(v, myreg, gl, twMl);
 #pragma unroll
             for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
         }

@@ -93,7 +93,12 @@ static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream
     return hipGetLastError();
 }
 
-template <typename R, int LOGMM> static size_t xstep_lds() { return (size_t)(XS / group_lanes<(1 << LOGMM)>()) * (1 << LOGMM) * sizeof(cplx_t<R>); }
+// transform buffers, then the MM-entry x twiddles and 2 MM + 1 fold twiddles
+template <typename R, int LOGMM> static size_t xstep_lds()
+{
+    constexpr size_t MM = (size_t)1 << LOGMM;
+    return (XS / group_lanes<(1 << LOGMM)>() + 1) * MM * sizeof(cplx_t<R>) + (2 * MM + 1) * sizeof(double2);
+}
 
 template <typename R, int LOGMM>
 static hipError_t launch_xstep(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
