@@ -770,7 +770,7 @@ __device__ __forceinline__ void lhsrhs_k(const Geo& g, int k, int which, bool me
     const double dt = g.dt;
     const double lap = (am2 + ak2) + an2;
     const double hlap = am2 + an2;
-    const double ilap16 = g.hypo / lap;
+    const double ilap16 = g.hypo != 0.0 ? g.hypo / lap : 0.0; // lap = 0 only where tvl is unused
     const double tvl = (g.dragall || k == 0) ? 1.0 - dt * ((lap / g.re + ilap16) - g.drag)
                                              : 1.0 - dt * (lap / g.re + ilap16);
     if (mean) {
