@@ -86,6 +86,33 @@ int mwf_nonlinear(mwf_handle *h, double *nlre, double *nlim);
 /* vel_mpt2phys (program/velocity.f90:84-90): mpt -> phys for any host mpt. */
 int mwf_mpt2phys(mwf_handle *h, const double *re, const double *im, double *phys);
 
+/* AB2 history vel_onl (program/velocity.f90:16,24-27,39), for a warm restart
+ * that continues AB2 instead of restarting with forward Euler: the state
+ * files of program/io.f90:257-313 hold vel_c only.  get returns the history
+ * in the mpt layout and first_step = 1 while the next step is a forward-Euler
+ * step (nst); set (after mwf_set_state) installs a saved history and clears
+ * nst, so that set_state + set_onl resumes bit-identically. */
+int mwf_get_onl(mwf_handle *h, double *re, double *im, int32_t *first_step);
+int mwf_set_onl(mwf_handle *h, const double *re, const double *im);
+
+/* io_save_Uspec (program/io.f90:345-395) on the device: with d = |spec| of
+ * var_mpt2spec(vel_c) over all wall-normal slots, m_max[|m|] (MM entries) and
+ * n_max[n] (NN entries) are the maxima of d over the other two indices. */
+int mwf_spectrum_max(mwf_handle *h, double *m_max, double *n_max);
+
+/* vel_history(vel_mpt2phys(vel_c), y, ans) (program/velocity.f90:164-180,
+ * program/io.f90:472-483) on the device: ans is (4, nh) column-major, per
+ * sample j the z position ji*Lz/Q (ji = (j-1)*Q/nh + 1) and velU, velV, velW
+ * (modes.M4.f90:83-108, modes.M6.f90:81-111) at x index 0 and height y.
+ * 1 <= nh <= Q/2.  Only the 4*nh results cross PCIe. */
+int mwf_vel_history(mwf_handle *h, int32_t nh, double y, double *ans);
+
+/* turb_sample(vel_mpt2phys(vel_c)) (program/turb.f90:37-47) on the device:
+ * out is (3, SN, SM) column-major, SM = P/dm, SN = Q/dn (parameters.f90:60-61),
+ * holding eposC (program/velocity.f90:128-142) at x = m*dm, z = n*dn.  The
+ * caller keeps the i_MT time history (turb.f90:43, the shift of tdata). */
+int mwf_turb_sample(mwf_handle *h, int32_t dm, int32_t dn, double *out);
+
 /* Wait for all queued work of the handle. */
 int mwf_sync(mwf_handle *h);
 

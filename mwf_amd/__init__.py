@@ -13,6 +13,10 @@ reference              here
 ``vel_nonlinear``      ``Velocity.vel_nonlinear()`` (returns vel_nl)
 ``vel_mpt2phys``       ``Velocity.vel_mpt2phys(re, im)``
 ``vel_energy``         ``Velocity.vel_energy()`` (of the current vel_c)
+``vel_onl`` (restart)  ``Velocity.get_onl()`` / ``set_onl``
+``io_save_Uspec``      ``Velocity.spectrum_max()``
+``vel_history``        ``Velocity.vel_history(nh, y)``
+``turb_sample``        ``Velocity.turb_sample(dm, dn)``
 =====================  ================================================
 
 mpt arrays are numpy float64 of shape ``(NN, M, K)`` -- the memory layout of
@@ -87,6 +91,11 @@ def lib():
         L.mwf_profile_enable.argtypes = [vp, ctypes.c_int32]
         L.mwf_profile_read.argtypes = [vp, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64)]
         L.mwf_kernel_bytes.argtypes = [vp, ctypes.c_int32, _dp]
+        L.mwf_get_onl.argtypes = [vp, _dp, _dp, ctypes.POINTER(ctypes.c_int32)]
+        L.mwf_set_onl.argtypes = [vp, _dp, _dp]
+        L.mwf_spectrum_max.argtypes = [vp, _dp, _dp]
+        L.mwf_vel_history.argtypes = [vp, ctypes.c_int32, ctypes.c_double, _dp]
+        L.mwf_turb_sample.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _dp]
         _LIB = L
     return _LIB
 
@@ -177,6 +186,40 @@ class Velocity:
 
     def sync(self):
         _check(lib().mwf_sync(self._h))
+
+    # AB2 history for a bit-identical warm restart (program/velocity.f90:16,24-27)
+    def get_onl(self):
+        re, im = np.empty(self.mpt_shape), np.empty(self.mpt_shape)
+        first = ctypes.c_int32()
+        _check(lib().mwf_get_onl(self._h, _p(re), _p(im), ctypes.byref(first)))
+        return re, im, bool(first.value)
+
+    def set_onl(self, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64).reshape(self.mpt_shape)
+        im = np.ascontiguousarray(im, dtype=np.float64).reshape(self.mpt_shape)
+        _check(lib().mwf_set_onl(self._h, _p(re), _p(im)))
+
+    # on-device diagnostics of the kept host I/O
+    def spectrum_max(self):
+        """io_save_Uspec (program/io.f90:345-395): (m_max[MM], n_max[NN])."""
+        m, n = np.empty(self.params.mm), np.empty(self.NN)
+        _check(lib().mwf_spectrum_max(self._h, _p(m), _p(n)))
+        return m, n
+
+    def vel_history(self, nh=32, y=0.0):
+        """vel_history (program/velocity.f90:164-180): array (nh, 4) of
+        z, u, v, w at x index 0 and height y."""
+        ans = np.empty((nh, 4))
+        _check(lib().mwf_vel_history(self._h, int(nh), float(y), _p(ans)))
+        return ans
+
+    def turb_sample(self, dm, dn):
+        """turb_sample (program/turb.f90:37-47): array (SM, SN, 3) of eposC at
+        x = m*dm, z = n*dn (the Fortran (3, SN, SM) layout)."""
+        sm, sn = self.P // dm, self.Q // dn
+        out = np.empty((sm, sn, 3))
+        _check(lib().mwf_turb_sample(self._h, int(dm), int(dn), _p(out)))
+        return out
 
     # per-kernel timing
     def profile(self, on=True):

@@ -963,7 +963,7 @@ __global__ void k_ereduce(const double* __restrict__ part, int n, int nmem, doub
 // ---------------------------------------------------------------------------
 constexpr int XS = 768;
 
-template <typename R, int LOGMM>
+template <typename R, int LOGMM, bool INV_ONLY>
 __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
                                               double2* __restrict__ onl, cplx_t<R>* __restrict__ U,
                                               const cplx_t<R>* __restrict__ twMM, const double2* __restrict__ twP,
@@ -1006,104 +1006,106 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
     for (int i = tid; i < MM; i += XS) twMl[i] = twMM[i];
     for (int i = tid; i <= 2 * MM; i += XS) twPl[i] = twP[i];
 
-    // ---------------- forward x transform + spec2mpt ----------------
+    if constexpr (!INV_ONLY) {
+        // ---------------- forward x transform + spec2mpt ----------------
 #pragma unroll 1
-    for (int comp = 0; comp < 3; comp++) {
-        if (k == 0 && comp == 1) continue;
-        const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
-        // opaque copy of the thread index: keeps item arithmetic inside the loop
-        int tidv = tid;
-        asm volatile("" : "+v"(tidv));
-        const CT* pan = nl_panel(g, NL, n0, slot);
+        for (int comp = 0; comp < 3; comp++) {
+            if (k == 0 && comp == 1) continue;
+            const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
+            // opaque copy of the thread index: keeps item arithmetic inside the loop
+            int tidv = tid;
+            asm volatile("" : "+v"(tidv));
+            const CT* pan = nl_panel(g, NL, n0, slot);
 #pragma unroll
-        for (int q = 0; q < LPT; q++) {
-            const int i = tidv + XS * q;
-            if (i >= nrows) continue;
-            const int x = i >> g.lgNT, t = i & (NT - 1);
-            const int a = x / 3, b = x - 3 * a;
-            reg[(t * 3 + b) * MM + pidx(a)] = ld_stream(pan + i);
+            for (int q = 0; q < LPT; q++) {
+                const int i = tidv + XS * q;
+                if (i >= nrows) continue;
+                const int x = i >> g.lgNT, t = i & (NT - 1);
+                const int a = x / 3, b = x - 3 * a;
+                reg[(t * 3 + b) * MM + pidx(a)] = ld_stream(pan + i);
+            }
+            __syncthreads();
+            if (seq_on) {
+                CT v[PL];
+#pragma unroll
+                for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
+                group_fft<CT, LOGMM, -1>(v, myreg, gl, twMl);
+#pragma unroll
+                for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < IPT; q++) {
+                const int it = tidv + XS * q;
+                if (it >= nitems) continue;
+                const int t = it >> LOGMM, r = it & (MM - 1);
+                const int n = n0 + t;
+                const CT* bt3 = reg + t * 3 * MM;
+                const CT F0 = bt3[pidx(r)];
+                double2 Xp = double2{(double)F0.x, (double)F0.y}, Xn = Xp;
+#pragma unroll
+                for (int b = 1; b < 3; b++) {
+                    const CT Fb = bt3[b * MM + pidx(r)];
+                    const double fr = (double)Fb.x, fi = (double)Fb.y;
+                    const double2 w1 = twPl[r * b];        // m = r:      times conj(w1)
+                    const double2 w2 = twPl[(MM - r) * b]; // m = r - MM: times w2
+                    Xp.x += fr * w1.x + fi * w1.y;
+                    Xp.y += fi * w1.x - fr * w1.y;
+                    Xn.x += fr * w2.x - fi * w2.y;
+                    Xn.y += fi * w2.x + fr * w2.y;
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (h == 1 && r == 0) continue;
+                    const double2 X = h == 0 ? Xp : Xn;
+                    const int mi = h == 0 ? r : MM - 1 + r;
+                    double am1, am2, an1, an2;
+                    xwave(g, mi, am1, am2);
+                    zwave(g, n, an1, an2);
+                    if (comp == 0) {
+                        psi[q][h] = X;
+                    } else if (comp == 1) {
+                        phi[q][h] = s2m_vterm(X, am2, an2);
+                    } else {
+                        const double2 u = psi[q][h], vh = phi[q][h];
+                        s2m_finish(k, mi, n, u, X, vh, am1, an1, ak1, psi[q][h], phi[q][h]);
+                    }
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (seq_on) {
-            CT v[PL];
-#pragma unroll
-            for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
-            group_fft<CT, LOGMM, -1>(v, myreg, gl, twMl);
-#pragma unroll
-            for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
-        }
-        __syncthreads();
+
+        // ---------------- update (program/velocity.f90:24-40) ----------------
+        const double dt = g.dt;
 #pragma unroll
         for (int q = 0; q < IPT; q++) {
-            const int it = tidv + XS * q;
+            const int it = tid + XS * q;
             if (it >= nitems) continue;
             const int t = it >> LOGMM, r = it & (MM - 1);
             const int n = n0 + t;
-            const CT* bt3 = reg + t * 3 * MM;
-            const CT F0 = bt3[pidx(r)];
-            double2 Xp = double2{(double)F0.x, (double)F0.y}, Xn = Xp;
-#pragma unroll
-            for (int b = 1; b < 3; b++) {
-                const CT Fb = bt3[b * MM + pidx(r)];
-                const double fr = (double)Fb.x, fi = (double)Fb.y;
-                const double2 w1 = twPl[r * b];        // m = r:      times conj(w1)
-                const double2 w2 = twPl[(MM - r) * b]; // m = r - MM: times w2
-                Xp.x += fr * w1.x + fi * w1.y;
-                Xp.y += fi * w1.x - fr * w1.y;
-                Xn.x += fr * w2.x - fi * w2.y;
-                Xn.y += fi * w2.x + fr * w2.y;
-            }
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 if (h == 1 && r == 0) continue;
-                const double2 X = h == 0 ? Xp : Xn;
                 const int mi = h == 0 ? r : MM - 1 + r;
                 double am1, am2, an1, an2;
                 xwave(g, mi, am1, am2);
                 zwave(g, n, an1, an2);
-                if (comp == 0) {
-                    psi[q][h] = X;
-                } else if (comp == 1) {
-                    phi[q][h] = s2m_vterm(X, am2, an2);
-                } else {
-                    const double2 u = psi[q][h], vh = phi[q][h];
-                    s2m_finish(k, mi, n, u, X, vh, am1, an1, ak1, psi[q][h], phi[q][h]);
+                const bool mean = (mi == 0 && n == 0);
+#pragma unroll
+                for (int which = 0; which < 2; which++) {
+                    if (which == 1 && k == 0) continue;
+                    const int slot = which == 0 ? k : g.K0 - 1 + k;
+                    const size_t idx = ((size_t)n * g.K + slot) * g.ldM + mi;
+                    const double2 nl = which == 0 ? psi[q][h] : phi[q][h];
+                    double lhs, rhs;
+                    lhsrhs_k(g, k, which, mean, am2, an2, ak2, lhs, rhs);
+                    const double2 cv = c[idx];
+                    const double2 ov = nst ? nl : onl[idx];
+                    double2 cn = ab2_update(cv, nl, ov, lhs, rhs, dt);
+                    if (mean && which == 0 && k == 0) cn.x = 0.0;
+                    c[idx] = cn;
+                    st_stream(&onl[idx], nl);
                 }
-            }
-        }
-        __syncthreads();
-    }
-
-    // ---------------- update (program/velocity.f90:24-40) ----------------
-    const double dt = g.dt;
-#pragma unroll
-    for (int q = 0; q < IPT; q++) {
-        const int it = tid + XS * q;
-        if (it >= nitems) continue;
-        const int t = it >> LOGMM, r = it & (MM - 1);
-        const int n = n0 + t;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            if (h == 1 && r == 0) continue;
-            const int mi = h == 0 ? r : MM - 1 + r;
-            double am1, am2, an1, an2;
-            xwave(g, mi, am1, am2);
-            zwave(g, n, an1, an2);
-            const bool mean = (mi == 0 && n == 0);
-#pragma unroll
-            for (int which = 0; which < 2; which++) {
-                if (which == 1 && k == 0) continue;
-                const int slot = which == 0 ? k : g.K0 - 1 + k;
-                const size_t idx = ((size_t)n * g.K + slot) * g.ldM + mi;
-                const double2 nl = which == 0 ? psi[q][h] : phi[q][h];
-                double lhs, rhs;
-                lhsrhs_k(g, k, which, mean, am2, an2, ak2, lhs, rhs);
-                const double2 cv = c[idx];
-                const double2 ov = nst ? nl : onl[idx];
-                double2 cn = ab2_update(cv, nl, ov, lhs, rhs, dt);
-                if (mean && which == 0 && k == 0) cn.x = 0.0;
-                c[idx] = cn;
-                st_stream(&onl[idx], nl);
             }
         }
     }
@@ -1179,4 +1181,94 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// On-device diagnostics of the kept host I/O, so that the 3/2-grid field
+// never has to leave the GPU for them.
+// ---------------------------------------------------------------------------
+
+// io_save_Uspec (program/io.f90:345-395): d = |spec(k, m, n)| of
+// var_mpt2spec(vel_c) (no laminar profile); n_max[n] = max over (k, m),
+// m_max[|m|] = max over (k, n).  Block n, threads over m.  The m maxima go
+// through an atomic max on the bit pattern, which orders non-negative doubles
+// like their values, so the result does not depend on the order of updates.
+__global__ __launch_bounds__(256) void k_specmax(Geo g, const double2* __restrict__ c,
+                                                 double* __restrict__ n_max,
+                                                 unsigned long long* __restrict__ m_max)
+{
+    const int n = blockIdx.x;
+    __shared__ double red[256];
+    double dn = 0.0;
+    for (int mi = threadIdx.x; mi < g.M; mi += blockDim.x) {
+        const double2* cn = c + (size_t)n * g.K * g.ldM;
+        double dm = 0.0;
+        for (int k = 0; k < g.K0; k++) {
+            const double2 ps = cn[(size_t)k * g.ldM + mi];
+            const double2 ph = k > 0 ? cn[(size_t)(g.K0 - 1 + k) * g.ldM + mi] : double2{0.0, 0.0};
+            double2 u, v, w;
+            mpt2spec_k(g, k, mi, n, ps, ph, false, u, v, w);
+            dm = fmax(dm, sqrt(u.x * u.x + u.y * u.y));
+            if (k > 0) dm = fmax(dm, sqrt(v.x * v.x + v.y * v.y));
+            dm = fmax(dm, sqrt(w.x * w.x + w.y * w.y));
+        }
+        dn = fmax(dn, dm);
+        const int ma = mi < g.MM ? mi : g.M - mi;
+        atomicMax(&m_max[ma], (unsigned long long)__double_as_longlong(dm));
+    }
+    red[threadIdx.x] = dn;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) n_max[n] = red[0];
+}
+
+// vel_history (program/velocity.f90:164-180, called from io.f90:472-483):
+// the u, v, w profiles at wall-normal position y of the physical field at x
+// index 0 and nh z-samples ji = (j*Q)/nh + 1.  hb holds the basis at y:
+// hb[k] = h_k(y), k < K0 (1, sin, cos, ...), hb[K0 - 1 + k] = g_k(y), k >= 1.
+// ans is (4, nh) column-major: z, u, v, w.  The sums run in modes.M4.f90:83-108
+// order (velU, velV, velW).
+__global__ void k_velhist(const double* __restrict__ phys, int Q, int KK, int K0, int nh, double lz,
+                          const double* __restrict__ hb, double* __restrict__ ans)
+{
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nh) return;
+    const int ji = (int)(((long long)j * Q) / nh) + 1;
+    const double* V = phys + (size_t)ji * KK;
+    double u = V[0], v = 0.0, w = V[2 * K0 - 1];
+    for (int k = 1; k < K0; k++) u = u + hb[k] * V[k];
+    for (int k = 1; k < K0; k++) v = k == 1 ? hb[K0 - 1 + k] * V[K0 - 1 + k] : v + hb[K0 - 1 + k] * V[K0 - 1 + k];
+    for (int k = 1; k < K0; k++) w = w + hb[k] * V[2 * K0 - 1 + k];
+    ans[4 * j + 0] = (double)ji * lz / Q;
+    ans[4 * j + 1] = u;
+    ans[4 * j + 2] = v;
+    ans[4 * j + 3] = w;
+}
+
+// turb_sample (program/turb.f90:37-47): eposC (program/velocity.f90:128-142)
+// of the physical field at x = m*dm, z = n*dn: the y-means of u^2, v^2, w^2,
+// nluw(u,u)(0) = u0 u0 + (u1 u1)/2 + ..., nlvv(v)(0) = v1**2/2 + ...
+// (modes.M4.f90:38-81).  out is (3, SN, SM) column-major.
+__global__ void k_eposc(const double* __restrict__ phys, int Q, int KK, int K0, int dm, int dn, int SM, int SN,
+                        double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= SM * SN) return;
+    const int m = i / SN, n = i - m * SN;
+    const double* V = phys + ((size_t)m * dm * Q + (size_t)n * dn) * KK;
+    double eu = V[0] * V[0], ev = 0.0, ew = V[2 * K0 - 1] * V[2 * K0 - 1];
+    for (int k = 1; k < K0; k++) eu = eu + (V[k] * V[k]) / 2;
+    for (int k = 1; k < K0; k++) {
+        const double t = V[K0 - 1 + k] * V[K0 - 1 + k] / 2;
+        ev = k == 1 ? t : ev + t;
+    }
+    for (int k = 1; k < K0; k++) ew = ew + (V[2 * K0 - 1 + k] * V[2 * K0 - 1 + k]) / 2;
+    out[3 * i + 0] = eu;
+    out[3 * i + 1] = ev;
+    out[3 * i + 2] = ew;
 }

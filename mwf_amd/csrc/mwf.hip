@@ -56,7 +56,7 @@ struct Plan {
     hipError_t (*zphys)(const Batch&, const Geo&, hipStream_t, const void*, double*, const void*, const void*) = nullptr;
     // fused forward + update + next inverse (MM <= 512); null otherwise
     hipError_t (*xstep)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
-                        const void*, const double2*, int) = nullptr;
+                        const void*, const double2*, int, int) = nullptr;
 };
 
 template <typename F> static void set_lds(F* f, size_t bytes)
@@ -102,12 +102,16 @@ template <typename R, int LOGMM> static size_t xstep_lds()
 
 template <typename R, int LOGMM>
 static hipError_t launch_xstep(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
-                               double2* onl, void* U, const void* twMM, const double2* twP, int nst)
+                               double2* onl, void* U, const void* twMM, const double2* twP, int nst, int inv_only)
 {
     using CT = cplx_t<R>;
     dim3 grid(g.NN / g.NTx, g.K0, nmem);
-    hipLaunchKernelGGL((k_xstep<R, LOGMM>), grid, dim3(XS), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL, c, onl,
-                       (CT*)U, (const CT*)twMM, twP, nst);
+    if (inv_only)
+        hipLaunchKernelGGL((k_xstep<R, LOGMM, true>), grid, dim3(XS), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
+                           c, onl, (CT*)U, (const CT*)twMM, twP, nst);
+    else
+        hipLaunchKernelGGL((k_xstep<R, LOGMM, false>), grid, dim3(XS), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
+                           c, onl, (CT*)U, (const CT*)twMM, twP, nst);
     return hipGetLastError();
 }
 
@@ -194,7 +198,8 @@ template <typename R> static void prepare_x(int lm)
     case L:                                           \
         set_lds(k_xinv<R, L>, x_lds<R, L>());        \
         set_lds(k_xfwd<R, L>, x_lds<R, L>());        \
-        set_lds(k_xstep<R, L>, xstep_lds<R, L>());   \
+        set_lds(k_xstep<R, L, false>, xstep_lds<R, L>()); \
+        set_lds(k_xstep<R, L, true>, xstep_lds<R, L>());  \
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
@@ -385,14 +390,18 @@ struct SizeClass {
 
     // one step of every member: vel_TS (program/velocity.f90:21-41).  With the
     // fused x kernel a step is k_z + k_xstep, k_xstep leaving U ready for the
-    // next step; k_xinv runs only when U is stale (first step, new state).
+    // next step.  When U is stale (first step, new state) it is rebuilt by the
+    // inverse half of k_xstep itself, so a run restarted from (c, onl) takes
+    // bit for bit the path of the run it continues.
     template <typename TIMED> int step(hipStream_t st, TIMED&& timed)
     {
-        if (!plan.xstep || !u_valid)
+        if (plan.xstep && !u_valid)
+            RCCHK(timed(0, [&] { return plan.xstep(bt, nmem, g0, st, NL, c, onl, U, twMM(), twPd, 0, 1); }));
+        else if (!plan.xstep)
             RCCHK(timed(0, [&] { return plan.xinv(bt, nmem, g0, st, c, U, nullptr, twMM(), twPd, 1); }));
         RCCHK(timed(1, [&] { return plan.zf(bt, nmem, g0, st, U, NL, twNN(), twQ()); }));
         if (plan.xstep) {
-            RCCHK(timed(4, [&] { return plan.xstep(bt, nmem, g0, st, NL, c, onl, U, twMM(), twPd, nst); }));
+            RCCHK(timed(4, [&] { return plan.xstep(bt, nmem, g0, st, NL, c, onl, U, twMM(), twPd, nst, 0); }));
             u_valid = true;
         } else {
             RCCHK(timed(2, [&] { return plan.xfwd(bt, nmem, g0, st, NL, c, onl, twMM(), twPd, 0, nst); }));
@@ -511,6 +520,8 @@ struct mwf_handle {
     std::unique_ptr<SizeClass> cls;
     double* phys = nullptr;
     size_t phys_elems = 0;
+    void* diag = nullptr;      // device scratch of the diagnostics
+    size_t diag_bytes = 0;
     double2* hstage = nullptr; // pinned host staging, one device mpt
     Profiler prof;
 };
@@ -522,6 +533,7 @@ static void destroy(mwf_handle* h)
     h->prof.release();
     h->cls.reset();
     if (h->phys) (void)hipFree(h->phys);
+    if (h->diag) (void)hipFree(h->diag);
     if (h->hstage) (void)hipHostFree(h->hstage);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
@@ -623,25 +635,132 @@ extern "C" int mwf_energy(mwf_handle* h, double* e)
     return 0;
 }
 
+static int ensure_phys(mwf_handle* h)
+{
+    const Geo& g = h->cls->g0;
+    const size_t np = (size_t)g.KK * g.Q * g.P;
+    if (h->phys_elems < np) {
+        if (h->phys) HIPCHK(hipFree(h->phys));
+        h->phys = nullptr;
+        h->phys_elems = 0;
+        HIPCHK(hipMalloc((void**)&h->phys, sizeof(double) * np));
+        h->phys_elems = np;
+    }
+    return 0;
+}
+
+static int ensure_diag(mwf_handle* h, size_t bytes)
+{
+    if (h->diag_bytes < bytes) {
+        if (h->diag) HIPCHK(hipFree(h->diag));
+        h->diag = nullptr;
+        h->diag_bytes = 0;
+        HIPCHK(hipMalloc(&h->diag, bytes));
+        h->diag_bytes = bytes;
+    }
+    return 0;
+}
+
+// vel_mpt2phys of a device mpt into h->phys (program/velocity.f90:84-90):
+// var_mpt2spec without the laminar profile, then tra_spec2phys.  U is
+// overwritten, so the next step re-derives it from c.
+static int phys_of(mwf_handle* h, const double2* src)
+{
+    SizeClass& k = *h->cls;
+    const Geo& g = k.g0;
+    RCCHK(ensure_phys(h));
+    k.u_valid = false;
+    hipError_t e = k.plan.xinv(k.bt, 1, g, h->st, src, k.U, nullptr, k.twMM(), k.twPd, 0);
+    if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
+    e = k.plan.zphys(k.bt, g, h->st, k.U, h->phys, k.twNN(), k.twQ());
+    if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
+    return 0;
+}
+
 extern "C" int mwf_mpt2phys(mwf_handle* h, const double* re, const double* im, double* phys)
 {
     if (!h || !re || !im || !phys) return fail(MWF_E_ARG, "null argument");
     SizeClass& k = *h->cls;
     const Geo& g = k.g0;
-    const size_t np = (size_t)g.KK * g.Q * g.P;
-    if (h->phys_elems < np) {
-        if (h->phys) HIPCHK(hipFree(h->phys));
-        h->phys = nullptr;
-        HIPCHK(hipMalloc((void**)&h->phys, sizeof(double) * np));
-        h->phys_elems = np;
-    }
     RCCHK(upload_mpt(h, re, im, k.cs));
-    k.u_valid = false;
-    hipError_t e = k.plan.xinv(k.bt, 1, g, h->st, k.cs, k.U, nullptr, k.twMM(), k.twPd, 0);
-    if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
-    e = k.plan.zphys(k.bt, g, h->st, k.U, h->phys, k.twNN(), k.twQ());
-    if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
-    HIPCHK(hipMemcpyAsync(phys, h->phys, sizeof(double) * np, hipMemcpyDeviceToHost, h->st));
+    RCCHK(phys_of(h, k.cs));
+    HIPCHK(hipMemcpyAsync(phys, h->phys, sizeof(double) * g.KK * g.Q * g.P, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+extern "C" int mwf_get_onl(mwf_handle* h, double* re, double* im, int32_t* first_step)
+{
+    if (!h || !re || !im || !first_step) return fail(MWF_E_ARG, "null argument");
+    *first_step = h->cls->nst;
+    return download_mpt(h, h->cls->onl, re, im);
+}
+
+extern "C" int mwf_set_onl(mwf_handle* h, const double* re, const double* im)
+{
+    if (!h || !re || !im) return fail(MWF_E_ARG, "null argument");
+    RCCHK(upload_mpt(h, re, im, h->cls->onl));
+    h->cls->nst = 0;
+    return 0;
+}
+
+extern "C" int mwf_spectrum_max(mwf_handle* h, double* m_max, double* n_max)
+{
+    if (!h || !m_max || !n_max) return fail(MWF_E_ARG, "null argument");
+    const Geo& g = h->cls->g0;
+    RCCHK(ensure_diag(h, sizeof(double) * (g.MM + g.NN)));
+    unsigned long long* dm = reinterpret_cast<unsigned long long*>(h->diag);
+    double* dn = reinterpret_cast<double*>(dm + g.MM);
+    HIPCHK(hipMemsetAsync(dm, 0, sizeof(unsigned long long) * g.MM, h->st));
+    hipLaunchKernelGGL(k_specmax, dim3(g.NN), dim3(256), 0, h->st, g, h->cls->c, dn, dm);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(m_max, dm, sizeof(double) * g.MM, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(n_max, dn, sizeof(double) * g.NN, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+extern "C" int mwf_vel_history(mwf_handle* h, int32_t nh, double y, double* ans)
+{
+    if (!h || !ans) return fail(MWF_E_ARG, "null argument");
+    const Geo& g = h->cls->g0;
+    if (nh < 1 || nh > g.Q / 2) return fail(MWF_E_ARG, "nh must be in [1, Q/2]");
+    // wall-normal bases at y (modes.M4.f90:83-108): h_k = sin/cos(k beta y) for
+    // odd/even k, g_k = cos/sin(k beta y) for odd/even k, from the host libm
+    std::vector<double> hb(2 * g.K0 - 1, 1.0);
+    for (int k = 1; k < g.K0; k++) {
+        const double a = k * g.beta * y;
+        hb[k] = (k & 1) ? std::sin(a) : std::cos(a);
+        hb[g.K0 - 1 + k] = (k & 1) ? std::cos(a) : std::sin(a);
+    }
+    RCCHK(phys_of(h, h->cls->c));
+    RCCHK(ensure_diag(h, sizeof(double) * (hb.size() + 4 * (size_t)nh)));
+    double* dhb = reinterpret_cast<double*>(h->diag);
+    double* dans = dhb + hb.size();
+    HIPCHK(hipMemcpyAsync(dhb, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, h->st));
+    const double lz = 2.0 * M_PI / g.gamma;
+    hipLaunchKernelGGL(k_velhist, dim3((nh + 63) / 64), dim3(64), 0, h->st, h->phys, g.Q, g.KK, g.K0, (int)nh, lz,
+                       dhb, dans);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ans, dans, sizeof(double) * 4 * nh, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+extern "C" int mwf_turb_sample(mwf_handle* h, int32_t dm, int32_t dn, double* out)
+{
+    if (!h || !out) return fail(MWF_E_ARG, "null argument");
+    const Geo& g = h->cls->g0;
+    if (dm < 1 || dn < 1 || dm > g.P || dn > g.Q) return fail(MWF_E_ARG, "sampling steps must be in [1, P] x [1, Q]");
+    const int SM = g.P / dm, SN = g.Q / dn;
+    RCCHK(phys_of(h, h->cls->c));
+    RCCHK(ensure_diag(h, sizeof(double) * 3 * (size_t)SM * SN));
+    double* dout = reinterpret_cast<double*>(h->diag);
+    const int npt = SM * SN;
+    hipLaunchKernelGGL(k_eposc, dim3((npt + 255) / 256), dim3(256), 0, h->st, h->phys, g.Q, g.KK, g.K0, (int)dm,
+                       (int)dn, SM, SN, dout);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * 3 * npt, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
 }

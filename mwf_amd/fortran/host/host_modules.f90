@@ -29,6 +29,8 @@ module parameters
    double precision, parameter :: d_beta = d_PI / 2d0, d_theta = d_thdeg / 180d0 * d_PI
    integer,          parameter :: i_3M = 3 * i_MM, i_3N = 3 * i_NN
    integer,          parameter :: i_Mp = i_3M, i_Np = i_NN, i_M1 = i_M - 1, i_K1 = i_K0 - 1
+   integer,          parameter :: i_DM = 6, i_DN = 6, i_MT = 10
+   integer,          parameter :: i_SM = i_3M / i_DM, i_SN = i_3N / i_DN, i_SMp = i_SM
    double precision :: tim_t
    integer          :: tim_step
 end module parameters

@@ -4,7 +4,10 @@
 ! i_save_rate2 steps to vel_energy.dat as '(2e20.12)', program/io.f90:460),
 ! then vel_TS, then t += dt.  The initial state comes from the seeded
 ! Gaussian generator of libmwf (mwf_ic_gaussian); the final state is written
-! to final_state.bin (Re then Im, raw doubles) after vel_sync_host.
+! to final_state.bin (Re then Im, raw doubles) after vel_sync_host, and the
+! device diagnostics of the final state to diag.bin (raw doubles): energy,
+! spectrum maxima m_, n_, the history at y = 0.25, the turb sample grid, then
+! the AB2 history vel_onl (Re, Im) and nst.
 !   usage: mwf_drive <nsteps> <seed> <rms> <k0>
 !
 program mwf_drive
@@ -17,6 +20,7 @@ program mwf_drive
    type(phys) :: vp
    type(mwf_params) :: p
    double precision :: E, scl, k0
+   double precision :: m_(0:i_MM-1), n_(0:i_NN-1), H(4, i_H), T(3, 0:i_SN-1, 0:i_SMp-1)
    integer :: nsteps, io_KE, uo
    integer(c_int64_t) :: seed
    character(64) :: arg
@@ -50,5 +54,13 @@ program mwf_drive
    call vel_sync_host()
    open(newunit=uo, file='final_state.bin', access='stream', form='unformatted', status='replace')
    write(uo) vel_c%Re, vel_c%Im
+   close(uo)
+   call vel_energy_dev(E)
+   call vel_spec_max(m_, n_)
+   call vel_history_dev(0.25d0, H)
+   call vel_turb_sample_dev(T)
+   call vel_sync_onl_host()
+   open(newunit=uo, file='diag.bin', access='stream', form='unformatted', status='replace')
+   write(uo) E, m_, n_, H, T, vel_onl%Re, vel_onl%Im, merge(1d0, 0d0, nst)
    close(uo)
 end program mwf_drive

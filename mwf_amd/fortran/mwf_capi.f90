@@ -82,6 +82,40 @@ module mwf_capi
          real(c_double), value :: scl, k0
          real(c_double), intent(out) :: re(*), im(*)
       end function mwf_ic_gaussian
+
+      integer(c_int) function mwf_get_onl(h, re, im, first_step) bind(C, name='mwf_get_onl')
+         import :: c_int, c_ptr, c_double, c_int32_t
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: re(*), im(*)
+         integer(c_int32_t), intent(out) :: first_step
+      end function mwf_get_onl
+
+      integer(c_int) function mwf_set_onl(h, re, im) bind(C, name='mwf_set_onl')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: re(*), im(*)
+      end function mwf_set_onl
+
+      integer(c_int) function mwf_spectrum_max(h, m_max, n_max) bind(C, name='mwf_spectrum_max')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: m_max(*), n_max(*)
+      end function mwf_spectrum_max
+
+      integer(c_int) function mwf_vel_history(h, nh, y, ans) bind(C, name='mwf_vel_history')
+         import :: c_int, c_ptr, c_double, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: nh
+         real(c_double), value :: y
+         real(c_double), intent(out) :: ans(*)
+      end function mwf_vel_history
+
+      integer(c_int) function mwf_turb_sample(h, dm, dn, out) bind(C, name='mwf_turb_sample')
+         import :: c_int, c_ptr, c_double, c_int32_t
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: dm, dn
+         real(c_double), intent(out) :: out(*)
+      end function mwf_turb_sample
    end interface
 
 contains

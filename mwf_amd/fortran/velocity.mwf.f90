@@ -32,7 +32,7 @@ module velocity
 
    type (phys) :: vel_p
    type (mpt), target :: vel_c
-   type (mpt)  :: vel_nl
+   type (mpt)  :: vel_nl, vel_onl
    logical :: nst
 
    type(c_ptr), private :: mwf_h = c_null_ptr
@@ -45,6 +45,7 @@ contains
       type(mwf_params) :: p
       call var_mpt_init(vel_c)
       call var_mpt_init(vel_nl)
+      call var_mpt_init(vel_onl)
       nst = .true.
       p%k0 = i_K0
       p%mm = i_MM
@@ -83,6 +84,65 @@ contains
       call mwf_check(mwf_get_state(mwf_h, vel_c%Re, vel_c%Im), 'mwf_get_state')
       dev_newer = .false.
    end subroutine vel_sync_host
+
+   ! the device holds the newest state: upload vel_c if the host changed it
+   subroutine vel_sync_device()
+      if (.not. host_newer) return
+      call mwf_check(mwf_set_state(mwf_h, vel_c%Re, vel_c%Im), 'mwf_set_state')
+      host_newer = .false.
+   end subroutine vel_sync_device
+
+   ! AB2 history for a warm restart (program/velocity.f90:16,24-27): fills
+   ! vel_onl and nst from the device, for io_save_state to write beside vel_c
+   subroutine vel_sync_onl_host()
+      integer(c_int32_t) :: first
+      call vel_sync_device()
+      call mwf_check(mwf_get_onl(mwf_h, vel_onl%Re, vel_onl%Im, first), 'mwf_get_onl')
+      nst = first /= 0
+   end subroutine vel_sync_onl_host
+
+   ! after io_load_state has filled vel_c and vel_onl from a warm-restart file:
+   ! continue AB2 with that history instead of a forward-Euler first step
+   subroutine vel_load_onl()
+      call mwf_check(mwf_set_state(mwf_h, vel_c%Re, vel_c%Im), 'mwf_set_state')
+      call mwf_check(mwf_set_onl(mwf_h, vel_onl%Re, vel_onl%Im), 'mwf_set_onl')
+      host_newer = .false.
+      dev_newer = .false.
+      nst = .false.
+   end subroutine vel_load_onl
+
+   ! vel_mpt2phys(vel_c) + vel_energy (program/io.f90:98-99) reduced on the
+   ! device: only the scalar crosses PCIe
+   subroutine vel_energy_dev(e)
+      double precision, intent(out) :: e
+      call vel_sync_device()
+      call mwf_check(mwf_energy(mwf_h, e), 'mwf_energy')
+   end subroutine vel_energy_dev
+
+   ! io_save_Uspec's spectrum maxima (program/io.f90:345-395) from the device
+   ! state: m_(0:i_MM-1), n_(0:i_NN-1)
+   subroutine vel_spec_max(m_, n_)
+      double precision, intent(out) :: m_(0:i_MM-1), n_(0:i_NN-1)
+      call vel_sync_device()
+      call mwf_check(mwf_spectrum_max(mwf_h, m_, n_), 'mwf_spectrum_max')
+   end subroutine vel_spec_max
+
+   ! vel_history(vel_mpt2phys(vel_c), y, ans) without moving the physical
+   ! field to the host (program/io.f90:472-483)
+   subroutine vel_history_dev(y, ans)
+      double precision, intent(in) :: y
+      double precision, intent(out) :: ans(4, i_H)
+      call vel_sync_device()
+      call mwf_check(mwf_vel_history(mwf_h, int(i_H, c_int32_t), y, ans), 'mwf_vel_history')
+   end subroutine vel_history_dev
+
+   ! turb_sample(vel_mpt2phys(vel_c)) (program/turb.f90:37-47): eposC on the
+   ! (i_SN, i_SMp) sample grid into t, the newest slice of turb's tdata
+   subroutine vel_turb_sample_dev(t)
+      double precision, intent(out) :: t(3, 0:i_SN-1, 0:i_SMp-1)
+      call vel_sync_device()
+      call mwf_check(mwf_turb_sample(mwf_h, int(i_DM, c_int32_t), int(i_DN, c_int32_t), t), 'mwf_turb_sample')
+   end subroutine vel_turb_sample_dev
 
    ! symmetry projections act on the host copy (program/velocity.f90:68-74)
    subroutine vel_imposesym()

@@ -73,3 +73,24 @@ def test_fortran_driver_matches_oracle(tmp_path):
     err = math.sqrt((np.sum((fre - ore) ** 2) + np.sum((fim - oim) ** 2)) / (np.sum(ore ** 2) + np.sum(oim ** 2)))
     print(f"[fortran] final state rel L2 {err:.2e}")
     assert err <= 1e-12
+
+    # the device diagnostics and the AB2 history through the Fortran bindings:
+    # a Python handle on the same final state gives the same numbers
+    d = np.fromfile(tmp_path / "diag.bin", dtype=np.float64)
+    v = mw.Velocity(p)
+    v.set_state(fre, fim)
+    sizes = [1, 32, 32, 4 * 32, 3 * 16 * 16, K * M * NN, K * M * NN, 1]
+    parts = np.split(d, np.cumsum(sizes)[:-1])
+    assert d.size == sum(sizes)
+    assert abs(parts[0][0] - o.energy()) <= 1e-12 * o.energy()
+    m_max, n_max = v.spectrum_max()
+    assert np.array_equal(parts[1], m_max) and np.array_equal(parts[2], n_max)
+    assert np.array_equal(parts[3], v.vel_history(32, 0.25).ravel())
+    assert np.array_equal(parts[4], v.turb_sample(6, 6).ravel())
+    w = mw.Velocity(p)
+    w.set_state(re, im)
+    w.vel_TS(nsteps)
+    hre, him, first = w.get_onl()
+    assert np.array_equal(parts[5].reshape(NN, M, K), hre) and np.array_equal(parts[6].reshape(NN, M, K), him)
+    assert parts[7][0] == 0.0 and not first
+    print("[fortran] device diagnostics and vel_onl match the Python handle")

@@ -1,0 +1,124 @@
+"""GPU: warm restart with the AB2 history, and the on-device diagnostics of
+the kept host I/O (io_save_Uspec, vel_history, turb_sample) against the oracle.
+
+The diagnostics are checked against quantities the oracle pins:
+var_mpt2spec (bit-identical to the compiled reference), vel_mpt2phys (DFT and
+numpy pins) and vel_energy (Parseval); and against closed forms of the
+wall-normal bases (h_0 = 1, h_k = sin/cos(k beta y), g_k = cos/sin(k beta y),
+modes.M4.f90:83-108, modes.M6.f90:81-111).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import mwf_amd as mw
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(k0, mm, nn, steps=5, seed=3):
+    p = mw.make_params(k0, mm, nn, mm * math.pi / 4, nn * math.pi / 4, 200.0, 0.02)
+    v = mw.Velocity(p)
+    v.set_state(*mw.ic_gaussian(p, seed, 0.2, 4.0))
+    v.vel_TS(steps)
+    o = oracle.Oracle(k0, mm, nn, p.alpha, p.gamma, p.re, p.dt, drag=p.drag, vdrag=p.vdrag,
+                      hypo=p.hypo, theta=p.theta, dragall=bool(p.dragall))
+    return p, v, o
+
+
+@pytest.mark.parametrize("k0,mm,nn", [(4, 32, 32), (6, 64, 32), (4, 512, 512)])
+def test_warm_restart_is_bit_identical(k0, mm, nn):
+    p = mw.make_params(k0, mm, nn, mm * math.pi / 4, nn * math.pi / 4, 200.0, 0.02)
+    a = mw.Velocity(p)
+    a.set_state(*mw.ic_gaussian(p, 1, 0.2, 4.0))
+    assert a.get_onl()[2], "a new state starts with a forward-Euler step"
+    a.vel_TS(30)
+    cre, cim = a.get_state()
+    hre, him, first = a.get_onl()
+    assert not first
+    a.vel_TS(20)
+    ref = a.get_state()
+
+    b = mw.Velocity(p)
+    b.set_state(cre, cim)
+    b.set_onl(hre, him)
+    b.vel_TS(20)
+    got = b.get_state()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+    # a restart from vel_c alone (the reference's state file) is not the same run
+    c = mw.Velocity(p)
+    c.set_state(cre, cim)
+    c.vel_TS(20)
+    cold = c.get_state()
+    diff = np.abs(cold[0] - ref[0]).max()
+    print(f"k0={k0} mm={mm} nn={nn}: resume with history bit-identical; without it max diff {diff:.2e}")
+    assert diff > 0.0
+
+
+@pytest.mark.parametrize("k0,mm,nn", [(4, 32, 32), (6, 64, 32), (4, 128, 64)])
+def test_spectrum_max_matches_oracle(k0, mm, nn):
+    p, v, o = setup(k0, mm, nn)
+    re, im = v.get_state()
+    sre, sim = o.mpt2spec(re, im)                   # (NN, M, KK), no laminar profile
+    d = np.sqrt(sre * sre + sim * sim)
+    n_ref = d.max(axis=(1, 2))
+    M = 2 * mm - 1
+    dm = d.max(axis=(0, 2))                         # over n and slots, per stored m
+    m_ref = np.array([dm[0]] + [max(dm[m], dm[M - m]) for m in range(1, mm)])
+    m_got, n_got = v.spectrum_max()
+    np.testing.assert_allclose(m_got, m_ref, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(n_got, n_ref, rtol=1e-14, atol=0)
+
+
+def bases(k0, y):
+    beta = math.pi / 2
+    h = [1.0] + [math.sin(k * beta * y) if k % 2 else math.cos(k * beta * y) for k in range(1, k0)]
+    g = [math.cos(k * beta * y) if k % 2 else math.sin(k * beta * y) for k in range(1, k0)]
+    return np.array(h), np.array(g)
+
+
+@pytest.mark.parametrize("k0,mm,nn", [(4, 32, 32), (6, 64, 32)])
+@pytest.mark.parametrize("y", [0.0, 0.3, -0.7])
+def test_vel_history_matches_oracle(k0, mm, nn, y):
+    p, v, o = setup(k0, mm, nn)
+    re, im = v.get_state()
+    phys = o.mpt2phys(re, im)                       # (P, Q, KK)
+    Q = 3 * nn
+    nh = 32
+    h, g = bases(k0, y)
+    got = v.vel_history(nh, y)
+    for j in range(nh):
+        ji = (j * Q) // nh + 1
+        V = phys[0, ji]
+        u = h @ V[:k0]
+        vv = g @ V[k0:2 * k0 - 1]
+        w = h @ V[2 * k0 - 1:]
+        if y == 0.0:  # sin(0) = 0, cos(0) = 1
+            assert np.isclose(u, V[0] + V[2:k0:2].sum(), rtol=1e-14, atol=1e-16)
+        exp = [ji * (2 * math.pi / p.gamma) / Q, u, vv, w]
+        np.testing.assert_allclose(got[j], exp, rtol=1e-12, atol=1e-14 * np.abs(phys).max())
+
+
+@pytest.mark.parametrize("k0,mm,nn", [(4, 32, 32), (6, 64, 32), (4, 512, 512)])
+def test_turb_sample_matches_oracle_and_energy(k0, mm, nn):
+    p, v, o = setup(k0, mm, nn)
+    # all points: the component energies add up to vel_energy (/(PQ))
+    full = v.turb_sample(1, 1)
+    e = v.vel_energy()
+    P, Q = 3 * mm, 3 * nn
+    assert abs(full.sum() / (P * Q) - e) <= 1e-12 * e
+    # sampled points against the oracle's physical field
+    re, im = v.get_state()
+    phys = o.mpt2phys(re, im)
+    dm, dn = 6, 6
+    got = v.turb_sample(dm, dn)
+    S = phys[::dm, ::dn][: P // dm, : Q // dn]
+    eu = S[..., 0] ** 2 + 0.5 * (S[..., 1:k0] ** 2).sum(-1)
+    ev = 0.5 * (S[..., k0:2 * k0 - 1] ** 2).sum(-1)
+    ew = S[..., 2 * k0 - 1] ** 2 + 0.5 * (S[..., 2 * k0:] ** 2).sum(-1)
+    ref = np.stack([eu, ev, ew], -1)
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-13 * ref.max())
+    print(f"k0={k0} mm={mm} nn={nn}: turb sample {got.shape}, sum/PQ vs energy ok")
