@@ -113,6 +113,17 @@ int mwf_vel_history(mwf_handle *h, int32_t nh, double y, double *ans);
  * caller keeps the i_MT time history (turb.f90:43, the shift of tdata). */
 int mwf_turb_sample(mwf_handle *h, int32_t dm, int32_t dn, double *out);
 
+/* var_doubleX / var_quadX (program/variables.f90:147-185, utils/doubleX.f90,
+ * utils/quadX.f90) on the device: dst (same k0, mm, nn; alpha = src alpha /
+ * factor, a box factor times longer in x) receives src's state with mode m
+ * moved to factor*m; modes beyond MM-1 are dropped.  factor is a power of
+ * two in [2, mm].  dst's next step is a first (forward-Euler) step. */
+int mwf_extend_x(mwf_handle *src, int32_t factor, mwf_handle *dst);
+
+/* utils/interpstate.f90:30-31 on the device: dst state = (1 - d) a + d b,
+ * for handles of equal (k0, mm, nn) (dst may be a or b). */
+int mwf_blend(mwf_handle *a, mwf_handle *b, double d, mwf_handle *dst);
+
 /* Wait for all queued work of the handle. */
 int mwf_sync(mwf_handle *h);
 

@@ -17,6 +17,8 @@ reference              here
 ``io_save_Uspec``      ``Velocity.spectrum_max()``
 ``vel_history``        ``Velocity.vel_history(nh, y)``
 ``turb_sample``        ``Velocity.turb_sample(dm, dn)``
+``var_doubleX/quadX``  ``Velocity.extend_x_into(dst, factor)``
+``interpstate``        ``Velocity.blend_into(a, b, d)``
 =====================  ================================================
 
 mpt arrays are numpy float64 of shape ``(NN, M, K)`` -- the memory layout of
@@ -96,6 +98,8 @@ def lib():
         L.mwf_spectrum_max.argtypes = [vp, _dp, _dp]
         L.mwf_vel_history.argtypes = [vp, ctypes.c_int32, ctypes.c_double, _dp]
         L.mwf_turb_sample.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _dp]
+        L.mwf_extend_x.argtypes = [vp, ctypes.c_int32, vp]
+        L.mwf_blend.argtypes = [vp, vp, ctypes.c_double, vp]
         _LIB = L
     return _LIB
 
@@ -220,6 +224,15 @@ class Velocity:
         out = np.empty((sm, sn, 3))
         _check(lib().mwf_turb_sample(self._h, int(dm), int(dn), _p(out)))
         return out
+
+    # large-box utilities (utils/doubleX.f90, quadX.f90, interpstate.f90)
+    def extend_x_into(self, dst, factor):
+        """dst (a Velocity with alpha / factor) gets this state with m -> factor*m."""
+        _check(lib().mwf_extend_x(self._h, int(factor), dst._h))
+
+    def blend_into(self, a, b, d):
+        """This handle's state becomes (1 - d) a + d b."""
+        _check(lib().mwf_blend(a._h, b._h, float(d), self._h))
 
     # per-kernel timing
     def profile(self, on=True):

@@ -1272,3 +1272,42 @@ __global__ void k_eposc(const double* __restrict__ phys, int Q, int KK, int K0, 
     out[3 * i + 1] = ev;
     out[3 * i + 2] = ew;
 }
+
+// ---------------------------------------------------------------------------
+// State utilities for large-box runs (utils/doubleX.f90, utils/quadX.f90,
+// utils/interpstate.f90).
+// ---------------------------------------------------------------------------
+
+// var_doubleX / var_quadX (program/variables.f90:147-185): the state of a box
+// `factor` times longer in x holding the same field, mode m -> factor*m.
+// Gather form over the destination: signed m_d = factor*m_s where divisible,
+// zero elsewhere (modes with |factor*m| > MM-1 are dropped).  One thread per
+// (n, slot, m_d), padding slots of ldM included.
+__global__ void k_extend_x(Geo g, const double2* __restrict__ src, double2* __restrict__ dst, int factor)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)g.NN * g.K * g.ldM;
+    if (i >= total) return;
+    const int mi = (int)(i % g.ldM);
+    const size_t row = i / g.ldM; // n * K + slot
+    double2 v = double2{0.0, 0.0};
+    if (mi < g.M) {
+        const int md = mi < g.MM ? mi : mi - g.M;
+        if (md % factor == 0) {
+            const int ms = md / factor;
+            v = src[row * g.ldM + (ms >= 0 ? ms : g.M + ms)];
+        }
+    }
+    dst[i] = v;
+}
+
+// utils/interpstate.f90:30-31: u := (1 - d) u1 + d u2, in that order
+__global__ void k_blend(size_t n, const double2* __restrict__ a, const double2* __restrict__ b, double d,
+                        double2* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 x = a[i], y = b[i];
+    out[i] = double2{(1.0 - d) * x.x + d * y.x, (1.0 - d) * x.y + d * y.y};
+}

@@ -765,6 +765,47 @@ extern "C" int mwf_turb_sample(mwf_handle* h, int32_t dm, int32_t dn, double* ou
     return 0;
 }
 
+static bool same_size(const mwf_params& a, const mwf_params& b)
+{
+    return a.k0 == b.k0 && a.mm == b.mm && a.nn == b.nn;
+}
+
+extern "C" int mwf_extend_x(mwf_handle* src, int32_t factor, mwf_handle* dst)
+{
+    if (!src || !dst || src == dst) return fail(MWF_E_ARG, "two distinct handles required");
+    if (factor < 2 || factor > src->p.mm || (factor & (factor - 1))) return fail(MWF_E_ARG, "factor must be a power of two in [2, mm]");
+    if (!same_size(src->p, dst->p)) return fail(MWF_E_ARG, "src and dst differ in k0, mm or nn");
+    const double want = src->p.alpha / factor;
+    if (std::fabs(dst->p.alpha - want) > 1e-12 * want) return fail(MWF_E_ARG, "dst alpha must be src alpha / factor");
+    SizeClass& s = *src->cls;
+    SizeClass& d = *dst->cls;
+    HIPCHK(hipStreamSynchronize(src->st));
+    const size_t total = (size_t)s.g0.NN * s.g0.K * s.g0.ldM;
+    hipLaunchKernelGGL(k_extend_x, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, dst->st, s.g0, s.c, d.c,
+                       (int)factor);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(dst->st));
+    d.nst = 1;
+    d.u_valid = false;
+    return 0;
+}
+
+extern "C" int mwf_blend(mwf_handle* a, mwf_handle* b, double d, mwf_handle* dst)
+{
+    if (!a || !b || !dst) return fail(MWF_E_ARG, "null handle");
+    if (!same_size(a->p, b->p) || !same_size(a->p, dst->p)) return fail(MWF_E_ARG, "handles differ in k0, mm or nn");
+    HIPCHK(hipStreamSynchronize(a->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    const size_t n = dst->cls->mpt_elems;
+    hipLaunchKernelGGL(k_blend, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, dst->st, n, a->cls->c, b->cls->c, d,
+                       dst->cls->c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(dst->st));
+    dst->cls->nst = 1;
+    dst->cls->u_valid = false;
+    return 0;
+}
+
 extern "C" int mwf_sync(mwf_handle* h)
 {
     if (!h) return fail(MWF_E_ARG, "null handle");

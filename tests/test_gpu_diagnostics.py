@@ -122,3 +122,51 @@ def test_turb_sample_matches_oracle_and_energy(k0, mm, nn):
     ref = np.stack([eu, ev, ew], -1)
     np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-13 * ref.max())
     print(f"k0={k0} mm={mm} nn={nn}: turb sample {got.shape}, sum/PQ vs energy ok")
+
+
+@pytest.mark.parametrize("factor", [2, 4])
+@pytest.mark.parametrize("k0", [4, 6])
+def test_extend_x_repeats_the_field(k0, factor):
+    """var_doubleX / var_quadX: a state of the box factor*Lx holding the same
+    field.  Pin: on the 3/2 grids (P points each) the new field at x_j equals
+    the old one at x_{factor*j mod P}, and the energy is unchanged, for a
+    state band-limited to |m| < MM/factor."""
+    mm, nn = 32, 32
+    L = 8 * math.pi
+    p = mw.make_params(k0, mm, nn, L, L, 200.0, 0.02)
+    q = mw.make_params(k0, mm, nn, factor * L, L, 200.0, 0.02)
+    re, im = mw.ic_gaussian(p, 5, 0.2, 8.0)
+    M = 2 * mm - 1
+    signed = np.where(np.arange(M) < mm, np.arange(M), np.arange(M) - M)
+    keep = (np.abs(signed) * factor <= mm - 1)[None, :, None]
+    re, im = re * keep, im * keep
+    src, dst = mw.Velocity(p), mw.Velocity(q)
+    src.set_state(re, im)
+    src.extend_x_into(dst, factor)
+    nre, nim = dst.get_state()
+    a = src.vel_mpt2phys(re, im)
+    b = dst.vel_mpt2phys(nre, nim)
+    P = 3 * mm
+    idx = (factor * np.arange(P)) % P
+    err = np.abs(b - a[idx]).max() / np.abs(a).max()
+    e0, e1 = src.vel_energy(), dst.vel_energy()
+    print(f"k0={k0} factor={factor}: tiled field max err {err:.2e}, energy {e0:.15g} -> {e1:.15g}")
+    assert err <= 1e-13
+    assert abs(e1 - e0) <= 1e-12 * e0
+    # dropped modes: nothing outside m = factor*m_s survives
+    nz = np.abs(nre).sum(axis=(0, 2)) + np.abs(nim).sum(axis=(0, 2))
+    assert np.all(nz[signed % factor != 0] == 0.0)
+
+
+def test_blend_is_the_interpstate_combination():
+    p = mw.make_params(4, 32, 32, 8 * math.pi, 8 * math.pi, 200.0, 0.02)
+    s1, s2 = mw.ic_gaussian(p, 1, 0.2, 4.0), mw.ic_gaussian(p, 2, 0.3, 4.0)
+    a, b, c = mw.Velocity(p), mw.Velocity(p), mw.Velocity(p)
+    a.set_state(*s1)
+    b.set_state(*s2)
+    d = 0.3
+    c.blend_into(a, b, d)
+    got = c.get_state()
+    for g, x, y in zip(got, s1, s2):
+        assert np.array_equal(g, (1.0 - d) * x + d * y)
+    assert c.get_onl()[2], "the blended state starts with a forward-Euler step"
