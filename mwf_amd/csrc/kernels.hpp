@@ -379,13 +379,35 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     constexpr int LP = lpad(NN);
     constexpr int H = NN / 2;
     constexpr int FT = XB * ZP * H / ZT;            // fold tasks per thread
+    constexpr int GL = group_lanes<NN>();           // lanes per transform group
+    // FP32: one group of GL lanes per sequence (below); FP64: block-wide
+    // ping-pong passes, whose butterflies need fewer live registers next to
+    // the 2 * 3 * KK double point values
+    constexpr bool GRP = sizeof(R) == 4;
     constexpr bool INB = fft_in_b<LOGNN>();
     static_assert(XB * ZP * H % ZT == 0, "fold tasks must tile the block");
     extern __shared__ unsigned char smem_raw[];
     CT* A = reinterpret_cast<CT*>(smem_raw);        // [SEQ][LP]
-    CT* B = A + SEQ * LP;
+    CT* B = A + SEQ * LP;                           // [SEQ][LP]
     CT* twNl = B + SEQ * LP;                        // [NN]
     CT* twQl = twNl + NN;                           // [2 NN]
+    // Group form: each sequence of a batch is transformed by its own group of
+    // GL lanes (group_fft: 8 points per lane, wave-local exchanges), so the
+    // waves run their transforms without block barriers and overlap each
+    // other's LDS and VALU work; batches alternate between A and B.  Ping-pong
+    // form: folds go to A, results come back in B when the pass count is odd.
+    auto seq_fft = [&](CT* buf, auto sign) {
+        const int gl = threadIdx.x % GL;
+        for (int s = threadIdx.x / GL; s < SEQ; s += ZT / GL) {
+            CT* sr = buf + s * LP;
+            CT v[NN / GL];
+#pragma unroll
+            for (int q = 0; q < NN / GL; q++) v[q] = sr[pidx(gl + GL * q)];
+            group_fft<CT, LOGNN, decltype(sign)::value>(v, sr, gl, twNl);
+#pragma unroll
+            for (int q = 0; q < NN / GL; q++) sr[pidx(gl + GL * q)] = v[q];
+        }
+    };
 
     const int tid = threadIdx.x;
     const int xl = tid >> LOGNN, r = tid & (NN - 1);
@@ -441,15 +463,22 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
         for (int f = 0; f < FT; f++) {
             const int tau = tid + ZT * f;
             const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
-            CT* seq = A + ((fx * ZP + fp) * 3) * LP;
+            CT* seq = (GRP && (BN & 1) ? B : A) + ((fx * ZP + fp) * 3) * LP;
 #pragma unroll
             for (int b = 0; b < 3; b++)
                 zfold_b<R>(seq + b * LP, b, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twQl);
         }
         if constexpr (BN + 1 < NBI) prefetch(BN + 1);
         __syncthreads();
-        pp_fft<CT, LOGNN, +1, ZT>(A, B, SEQ, twNl);
-        const CT* F = INB ? B : A;
+        CT* F;
+        if constexpr (GRP) {
+            F = BN & 1 ? B : A;
+            seq_fft(F, std::integral_constant<int, +1>{});
+            __syncthreads();
+        } else {
+            pp_fft<CT, LOGNN, +1, ZT>(A, B, SEQ, twNl);
+            F = INB ? B : A;
+        }
         // consume: thread (xl, a = r)
         static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
@@ -466,7 +495,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
                 });
             }
         });
-        if constexpr (!INB) __syncthreads();
+        if constexpr (!GRP && !INB) __syncthreads();
     });
 #pragma unroll
     for (int b = 0; b < 3; b++) G::yterms(vb[b], an[b], (R)g.beta);
@@ -475,6 +504,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     const R scale = (R)g.scale;
     static_for<0, NBO>([&](auto ri) {
         constexpr int RD = decltype(ri)::value;
+        CT* F = GRP && ((NBI + RD) & 1) ? B : A;
         static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
             constexpr int c1 = 2 * (RD * ZP + p), c2 = c1 + 1;
@@ -483,13 +513,18 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
                 for (int b = 0; b < 3; b++) {
                     R v2 = (R)0;
                     if constexpr (c2 < KK) v2 = an[b][c2];
-                    A[((xl * ZP + p) * 3 + b) * LP + pidx(r)] = CT{an[b][c1], v2};
+                    F[((xl * ZP + p) * 3 + b) * LP + pidx(r)] = CT{an[b][c1], v2};
                 }
             }
         });
         __syncthreads();
-        pp_fft<CT, LOGNN, -1, ZT>(A, B, SEQ, twNl);
-        const CT* F = INB ? B : A;
+        if constexpr (GRP) {
+            seq_fft(F, std::integral_constant<int, -1>{});
+            __syncthreads();
+        } else {
+            pp_fft<CT, LOGNN, -1, ZT>(A, B, SEQ, twNl);
+            F = INB ? B : A;
+        }
         static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
             constexpr int c1 = 2 * (RD * ZP + p), c2 = c1 + 1;
@@ -517,7 +552,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
                 }
             }
         });
-        if constexpr (!INB) __syncthreads();
+        if constexpr (!GRP && !INB) __syncthreads();
     });
 }
 
