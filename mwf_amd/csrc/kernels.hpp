@@ -666,6 +666,19 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
             const int rm = (NN - r) & (NN - 1);
             const int x = x0 + xl;
             const bool valid = x < g.P;
+            // FP32: the partial sums of the earlier phases are loaded before the
+            // transform, so that their latency hides behind it (FP64 has no
+            // registers to spare for them and loads them after)
+            constexpr bool PRE = sizeof(R) == 4;
+            CT o1[ZO_], o2[ZO_];
+            if (PRE && b > 0 && valid) {
+                static_for<0, ZO_>([&](auto pi) {
+                    constexpr int p = decltype(pi)::value;
+                    constexpr int c1 = 2 * (RD * ZO_ + p), c2 = c1 + 1;
+                    if constexpr (c1 < KK) o1[p] = NL[nidx(g, x, c1, r)];
+                    if constexpr (c2 < KK) o2[p] = NL[nidx(g, x, c2, r)];
+                });
+            }
             static_for<0, ZO_>([&](auto pi) {
                 constexpr int p = decltype(pi)::value;
                 constexpr int c1 = 2 * (RD * ZO_ + p), c2 = c1 + 1;
@@ -691,13 +704,11 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
                         X2 = cmul(X2, w);
                     }
                     if (valid) {
-                        CT* o1 = &NL[nidx(g, x, c1, r)];
-                        if (b > 0) X1 = cadd(*o1, X1);
-                        *o1 = b == 2 ? cscale(X1, scale) : X1;
+                        if (b > 0) X1 = cadd(PRE ? o1[p] : NL[nidx(g, x, c1, r)], X1);
+                        NL[nidx(g, x, c1, r)] = b == 2 ? cscale(X1, scale) : X1;
                         if constexpr (c2 < KK) {
-                            CT* o2 = &NL[nidx(g, x, c2, r)];
-                            if (b > 0) X2 = cadd(*o2, X2);
-                            *o2 = b == 2 ? cscale(X2, scale) : X2;
+                            if (b > 0) X2 = cadd(PRE ? o2[p] : NL[nidx(g, x, c2, r)], X2);
+                            NL[nidx(g, x, c2, r)] = b == 2 ? cscale(X2, scale) : X2;
                         }
                     }
                 }
