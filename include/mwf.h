@@ -146,7 +146,7 @@ int mwf_ic_gaussian(const mwf_params *p, uint64_t seed, double scl, double k0,
 
 /* Per-kernel timing with HIP events on the handle's stream.  kernel ids:
  * 0 = x-inverse, 1 = z fused, 2 = x-forward + update, 3 = energy,
- * 4 = fused x-forward + update + next x-inverse (MM <= 512).  total_ms is the
+ * 4 = fused x-forward + update + next x-inverse.  total_ms is the
  * summed event time since profiling was enabled. */
 int mwf_profile_enable(mwf_handle *h, int32_t on);
 int mwf_profile_read(mwf_handle *h, int32_t kernel, double *total_ms, int64_t *count);

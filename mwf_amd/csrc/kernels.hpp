@@ -1002,15 +1002,17 @@ __global__ void k_ereduce(const double* __restrict__ part, int n, int nmem, doub
 // Fused x kernel of one step: forward x transform of NL, spec2mpt, AB2 and
 // implicit update of c and onl, then the inverse x transform of the updated
 // state for the next step's z kernel (mpt2spec + laminar + d/dx), so c is read
-// once per step.  Block (n-tile, y-mode k), XS = 768 threads: each group of
-// MM/8 lanes owns one of the 3*NT sequences (t, b), 8 points per lane, and
-// transforms it with group_fft (no block barrier inside a transform); items
-// (t, r), up to 3 per thread, hold psi_k / phi_k at m = r and m = r - MM.
+// once per step.  Block (n-tile, y-mode k), XS = 3 * NT * G threads: each
+// group of G = min(64, MM/8) lanes owns one of the 3*NT sequences (t, b), 8
+// points per lane (16 at MM = 1024), and transforms it with group_fft (no
+// block barrier inside a transform); items (t, r), up to XTILE/XS per thread,
+// hold psi_k / phi_k at m = r and m = r - MM.  768 threads for MM <= 512,
+// 384 at MM = 1024.
 // ---------------------------------------------------------------------------
-constexpr int XS = 768;
+template <int LOGMM> constexpr int xs_of() { return 3 * (XTILE >> LOGMM) * group_lanes<(1 << LOGMM)>(); }
 
 template <typename R, int LOGMM, bool INV_ONLY>
-__global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
+__global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
                                               double2* __restrict__ onl, cplx_t<R>* __restrict__ U,
                                               const cplx_t<R>* __restrict__ twMM, const double2* __restrict__ twP,
                                               int nst)
@@ -1019,7 +1021,7 @@ __global__ __launch_bounds__(XS) void k_xstep(Batch bt, const cplx_t<R>* __restr
     constexpr int MM = 1 << LOGMM;
     constexpr int G = group_lanes<MM>();
     constexpr int PL = MM / G;
-    static_assert(PL == 8, "k_xstep covers MM <= 512");
+    constexpr int XS = xs_of<LOGMM>();
     constexpr int IPT = (XTILE + XS - 1) / XS; // items per thread
     constexpr int LPT = 3 * XTILE / XS;        // rows per thread in the cooperative load/store
     extern __shared__ unsigned char smem_raw[];

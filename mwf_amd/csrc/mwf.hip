@@ -54,7 +54,7 @@ struct Plan {
                        const double2*, int, int) = nullptr;
     hipError_t (*zf)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*, const void*) = nullptr;
     hipError_t (*zphys)(const Batch&, const Geo&, hipStream_t, const void*, double*, const void*, const void*) = nullptr;
-    // fused forward + update + next inverse (MM <= 512); null otherwise
+    // fused forward + update + next inverse
     hipError_t (*xstep)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
                         const void*, const double2*, int, int) = nullptr;
 };
@@ -97,7 +97,7 @@ static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream
 template <typename R, int LOGMM> static size_t xstep_lds()
 {
     constexpr size_t MM = (size_t)1 << LOGMM;
-    return (XS / group_lanes<(1 << LOGMM)>() + 1) * MM * sizeof(cplx_t<R>) + (2 * MM + 1) * sizeof(double2);
+    return (xs_of<LOGMM>() / group_lanes<(1 << LOGMM)>() + 1) * MM * sizeof(cplx_t<R>) + (2 * MM + 1) * sizeof(double2);
 }
 
 template <typename R, int LOGMM>
@@ -107,10 +107,10 @@ static hipError_t launch_xstep(const Batch& bt, int nmem, const Geo& g, hipStrea
     using CT = cplx_t<R>;
     dim3 grid(g.NN / g.NTx, g.K0, nmem);
     if (inv_only)
-        hipLaunchKernelGGL((k_xstep<R, LOGMM, true>), grid, dim3(XS), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
+        hipLaunchKernelGGL((k_xstep<R, LOGMM, true>), grid, dim3(xs_of<LOGMM>()), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
                            c, onl, (CT*)U, (const CT*)twMM, twP, nst);
     else
-        hipLaunchKernelGGL((k_xstep<R, LOGMM, false>), grid, dim3(XS), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
+        hipLaunchKernelGGL((k_xstep<R, LOGMM, false>), grid, dim3(xs_of<LOGMM>()), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
                            c, onl, (CT*)U, (const CT*)twMM, twP, nst);
     return hipGetLastError();
 }
@@ -168,6 +168,9 @@ template <typename R> static void pick_x(int lm, Plan& p)
     case 10:
         p.xinv = launch_xinv<R, 10>;
         p.xfwd = launch_xfwd<R, 10>;
+        // the fused kernel at MM = 1024 pays off in FP64 (2.16 against 2.66 ms at
+        // M6 2048^2); in FP32 its 16-point lanes cost occupancy and it is slower
+        if constexpr (std::is_same<R, double>::value) p.xstep = launch_xstep<R, 10>;
         break;
     default: break;
     }
@@ -206,6 +209,10 @@ template <typename R> static void prepare_x(int lm)
     case 10:
         set_lds(k_xinv<R, 10>, x_lds<R, 10>());
         set_lds(k_xfwd<R, 10>, x_lds<R, 10>());
+        if constexpr (std::is_same<R, double>::value) {
+            set_lds(k_xstep<R, 10, false>, xstep_lds<R, 10>());
+            set_lds(k_xstep<R, 10, true>, xstep_lds<R, 10>());
+        }
         break;
     default: break;
     }
