@@ -36,6 +36,7 @@ IC = dict(seed=2, scl=0.3, k0=8.0)
 def run_gpu(mw, precision, steps, warmup, ic, workload=WORKLOAD):
     p = mw.make_params(precision=precision, **workload)
     v = mw.Velocity(p)
+    plan = v.plan_info()
     v.set_state(*ic)
     v.vel_TS(warmup)
     v.sync()
@@ -57,7 +58,7 @@ def run_gpu(mw, precision, steps, warmup, ic, workload=WORKLOAD):
     e = v.vel_energy()
     v.close()
     return dict(seconds=t1 - t0, steps_per_s=steps / (t1 - t0), kernels=kern, finite=finite,
-                energy=e)
+                energy=e, plan=plan)
 
 
 def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20, track_every=50):
@@ -271,6 +272,7 @@ def main():
     ap.add_argument("--m6-steps", type=int, default=100)
     ap.add_argument("--m6-err-steps", type=int, default=100)
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="rule-based kernel plans instead of timed ones")
     ap.add_argument("--cfg5-steps", type=int, default=200)
     ap.add_argument("--calib-copy", type=int, default=0, help="only run a MiB-sized device copy")
     args = ap.parse_args()
@@ -283,6 +285,8 @@ def main():
         sys.exit(2)
 
     import mwf_amd as mw
+    if not args.no_tune:
+        mw.autotune(1)  # timed plan per size class at handle creation (tra_precompute's role)
     p = mw.make_params(**WORKLOAD)
     ic = mw.ic_gaussian(p, IC["seed"], IC["scl"], IC["k0"])
 
@@ -316,6 +320,7 @@ def main():
                           "achieved_GBs": step_bytes * fp64["steps_per_s"] / 1e9,
                           "frac": step_bytes * fp64["steps_per_s"] / 1e9 / PEAK_HBM_GBS},
         "kernels_ms": {n: round(k["avg_ms"], 5) for n, k in kern.items()},
+        "plan": fp64["plan"],
         "finite": fp64["finite"],
     }
     tr = traffic_summary()

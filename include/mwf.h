@@ -124,6 +124,20 @@ int mwf_extend_x(mwf_handle *src, int32_t factor, mwf_handle *dst);
  * for handles of equal (k0, mm, nn) (dst may be a or b). */
 int mwf_blend(mwf_handle *a, mwf_handle *b, double d, mwf_handle *dst);
 
+/* Autotuning, in place of tra_precompute's FFTW_PATIENT planning
+ * (program/transform.fftw5.f90:33-59).  Process-wide mode for handles and
+ * ensembles created afterwards: 0 (default) = rule-based plans; 1 = the first
+ * handle or ensemble class of each (precision, k0, mm, nn) times its
+ * candidate kernel plans at creation (a few steps each, on its zero state)
+ * and keeps the fastest, and later ones of that size reuse the choice, so
+ * results stay reproducible within a process; 2..5 = force plan choice
+ * mode - 2 where that alternative exists (for tests and plan comparisons). */
+int mwf_autotune(int32_t mode);
+/* The plan a handle runs: bit 0 = phase-sequential z kernel instead of the
+ * three-phase one, bit 1 = separate x-inverse and x-forward kernels instead
+ * of the fused x kernel. */
+int mwf_plan_info(mwf_handle *h, int32_t *choice);
+
 /* Wait for all queued work of the handle. */
 int mwf_sync(mwf_handle *h);
 

@@ -100,6 +100,8 @@ def lib():
         L.mwf_turb_sample.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _dp]
         L.mwf_extend_x.argtypes = [vp, ctypes.c_int32, vp]
         L.mwf_blend.argtypes = [vp, vp, ctypes.c_double, vp]
+        L.mwf_autotune.argtypes = [ctypes.c_int32]
+        L.mwf_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         _LIB = L
     return _LIB
 
@@ -112,6 +114,16 @@ def _check(rc):
 def _p(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_dp)
+
+
+def autotune(mode=1):
+    """Process-wide plan selection for handles created afterwards (mwf_autotune),
+    the role of tra_precompute's FFTW_PATIENT planning: 0 = rule-based plans,
+    1 = timed choice per size, 2..5 = force plan bits mode - 2."""
+    _check(lib().mwf_autotune(int(mode)))
+
+
+PLAN_ZB, PLAN_SPLIT_X = 1, 2
 
 
 def dims(p):
@@ -224,6 +236,12 @@ class Velocity:
         out = np.empty((sm, sn, 3))
         _check(lib().mwf_turb_sample(self._h, int(dm), int(dn), _p(out)))
         return out
+
+    def plan_info(self):
+        """Plan bits: PLAN_ZB (phase-sequential z kernel), PLAN_SPLIT_X (separate x kernels)."""
+        ch = ctypes.c_int32()
+        _check(lib().mwf_plan_info(self._h, ctypes.byref(ch)))
+        return ch.value
 
     # large-box utilities (utils/doubleX.f90, quadX.f90, interpstate.f90)
     def extend_x_into(self, dst, factor):

@@ -13,7 +13,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <array>
+#include <atomic>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <tuple>
@@ -57,6 +60,9 @@ struct Plan {
     // fused forward + update + next inverse
     hipError_t (*xstep)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
                         const void*, const double2*, int, int) = nullptr;
+    // alternative z kernel the autotuner may pick (k_zb where k_z also runs)
+    hipError_t (*zf_alt)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*,
+                         const void*) = nullptr;
 };
 
 template <typename F> static void set_lds(F* f, size_t bytes)
@@ -186,6 +192,11 @@ template <typename R, int K0> static void pick_z(int ln, Plan& p)
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
+    default: break;
+    }
+    switch (ln) { // phase-sequential alternatives for the autotuner
+    case 8: p.zf_alt = launch_zb<R, K0, 8>; break;
+    case 9: p.zf_alt = launch_zb<R, K0, 9>; break;
     case 10: // one x-row per 1024-thread block: the three fold phases in sequence
         p.zf = launch_zb<R, K0, 10>;
         p.zphys = launch_zphys<R, K0, 10>;
@@ -228,6 +239,11 @@ template <typename R, int K0> static void prepare_z(int ln)
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
+    default: break;
+    }
+    switch (ln) {
+    case 8: set_lds(k_zb<R, K0, 8>, zb_lds_bytes<R>()); break;
+    case 9: set_lds(k_zb<R, K0, 9>, zb_lds_bytes<R>()); break;
     case 10:
         set_lds(k_zb<R, K0, 10>, zb_lds_bytes<R>());
         set_lds(k_zphys<R, K0, 10>, zphys_lds<R, 10>());
@@ -330,6 +346,22 @@ template <typename T> static int dev_upload(T** d, const T* h, size_t n)
 // ---------------------------------------------------------------------------
 // size class: members sharing (K0, MM, NN, precision)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Autotuner: the role of tra_precompute's FFTW_PATIENT planning
+// (program/transform.fftw5.f90:33-59).  Mode 0 (default): the rule-based plan
+// (k_z for NN <= 512, the fused x kernel wherever it exists).  Mode 1: each
+// new size class times its candidate plans on its own buffers (zero state,
+// 2 + 3 steps each) and keeps the fastest; the choice is cached per
+// (precision, K0, MM, NN) for the process, so every handle and ensemble class
+// of one size runs the same kernels.  Modes 2..5 force choice mode - 2 (where
+// the alternative exists), for tests and plan comparisons.  Choice bits: 1 =
+// phase-sequential z kernel (k_zb) instead of k_z, 2 = x-inverse + x-forward
+// instead of the fused x kernel.
+// ---------------------------------------------------------------------------
+static std::atomic<int> g_tune{0};
+static std::mutex g_tune_mu;
+static std::map<std::array<int, 4>, int> g_tune_cache;
+
 struct SizeClass {
     int nmem = 0;
     int prec = MWF_FP64;
@@ -347,6 +379,8 @@ struct SizeClass {
     float2 *twMMf = nullptr, *twNNf = nullptr, *twQf = nullptr;
     int nst = 1;
     bool u_valid = false; // U holds the x-inverse of the current c (with the laminar profile)
+    Plan base;            // the rule-based plan; plan = base with the chosen alternatives
+    int choice = 0;       // autotuner choice bits (see g_tune_cache)
 
     const void* twMM() const { return prec == MWF_MIXED ? (const void*)twMMf : (const void*)twMMd; }
     const void* twNN() const { return prec == MWF_MIXED ? (const void*)twNNf : (const void*)twNNd; }
@@ -365,7 +399,7 @@ struct SizeClass {
         prec = ps[0].precision;
         g0 = geo_of(&ps[0]);
         for (auto& p : ps) geos_h.push_back(geo_of(&p));
-        plan = plan_for(prec, g0.K0, g0.MM, g0.NN);
+        plan = base = plan_for(prec, g0.K0, g0.MM, g0.NN);
         if (!plan.xinv || !plan.xfwd || !plan.zf || !plan.zphys) return fail(MWF_E_ARG, "no kernel for this size");
         const size_t rsz = prec == MWF_MIXED ? sizeof(float2) : sizeof(double2);
         mpt_elems = (size_t)g0.NN * g0.K * g0.ldM;
@@ -392,6 +426,76 @@ struct SizeClass {
         HIPCHK(hipMemsetAsync(c, 0, sizeof(double2) * mpt_elems * nmem, st));
         HIPCHK(hipMemsetAsync(onl, 0, sizeof(double2) * mpt_elems * nmem, st));
         HIPCHK(hipStreamSynchronize(st));
+        const int mode = g_tune.load();
+        if (mode == 1) RCCHK(tune(st));
+        else if (mode >= 2) apply(mode - 2);
+        return 0;
+    }
+
+    void apply(int ch)
+    {
+        choice = ch;
+        plan = base;
+        if ((ch & 1) && base.zf_alt) plan.zf = base.zf_alt;
+        if (ch & 2) plan.xstep = nullptr;
+    }
+
+    int reset_state(hipStream_t st)
+    {
+        HIPCHK(hipMemsetAsync(c, 0, sizeof(double2) * mpt_elems * nmem, st));
+        HIPCHK(hipMemsetAsync(onl, 0, sizeof(double2) * mpt_elems * nmem, st));
+        HIPCHK(hipStreamSynchronize(st));
+        nst = 1;
+        u_valid = false;
+        return 0;
+    }
+
+    int tune(hipStream_t st)
+    {
+        const std::array<int, 4> key{prec, g0.K0, g0.MM, g0.NN};
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            apply(it->second);
+            return 0;
+        }
+        std::vector<int> cand;
+        for (int z = 0; z <= (base.zf_alt ? 1 : 0); z++) {
+            if (base.xstep) cand.push_back(z);
+            cand.push_back(z | 2);
+        }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        auto run = [&](int, auto&& f) -> int {
+            const hipError_t e = f();
+            return e == hipSuccess ? 0 : fail(MWF_E_HIP, hipGetErrorString(e));
+        };
+        int best = cand[0];
+        float best_ms = -1.0f;
+        int rc = 0;
+        for (int ch : cand) {
+            apply(ch);
+            if ((rc = reset_state(st))) break;
+            for (int w = 0; w < 2 && !rc; w++) rc = step(st, run);
+            if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = fail(MWF_E_HIP, "hipEventRecord");
+            for (int w = 0; w < 3 && !rc; w++) rc = step(st, run);
+            if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = fail(MWF_E_HIP, "hipEventRecord");
+            float ms = 0.0f;
+            if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = fail(MWF_E_HIP, "hipEventElapsedTime");
+            if (rc) break;
+            if (best_ms < 0.0f || ms < best_ms) {
+                best_ms = ms;
+                best = ch;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        apply(best);
+        RCCHK(reset_state(st));
+        g_tune_cache[key] = best;
         return 0;
     }
 
@@ -810,6 +914,21 @@ extern "C" int mwf_blend(mwf_handle* a, mwf_handle* b, double d, mwf_handle* dst
     HIPCHK(hipStreamSynchronize(dst->st));
     dst->cls->nst = 1;
     dst->cls->u_valid = false;
+    return 0;
+}
+
+extern "C" int mwf_autotune(int32_t mode)
+{
+    if (mode < 0 || mode > 5) return fail(MWF_E_ARG, "mode must be 0 (rules), 1 (timed) or 2..5 (forced plan)");
+    g_tune.store(mode);
+    return 0;
+}
+
+extern "C" int mwf_plan_info(mwf_handle* h, int32_t* choice)
+{
+    if (!h || !choice) return fail(MWF_E_ARG, "null argument");
+    const SizeClass& k = *h->cls;
+    *choice = ((k.base.zf_alt && k.plan.zf == k.base.zf_alt) ? 1 : 0) | (k.plan.xstep ? 0 : 2);
     return 0;
 }
 

@@ -1,0 +1,60 @@
+"""GPU: the autotuner's candidate plans (mwf_autotune) all keep parity, and
+the timed choice is one of them and is reused for a second handle of the
+same size.  Plans: bit 1 = phase-sequential z kernel, bit 2 = separate
+x-inverse and x-forward kernels."""
+import math
+
+import numpy as np
+import pytest
+
+import mwf_amd as mw
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.concatenate([np.ravel(x) for x in a])
+    b = np.concatenate([np.ravel(x) for x in b])
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture
+def tuning():
+    yield mw.autotune
+    mw.autotune(0)
+
+
+@pytest.mark.parametrize("k0,mm,nn", [(4, 32, 512), (6, 64, 256)])
+@pytest.mark.parametrize("choice", [0, 1, 2, 3])
+def test_every_plan_keeps_parity(tuning, k0, mm, nn, choice):
+    tuning(2 + choice)
+    p = mw.make_params(k0, mm, nn, mm * math.pi / 4, nn * math.pi / 4, 200.0, 0.02)
+    v = mw.Velocity(p)
+    assert v.plan_info() == choice
+    o = oracle.Oracle(k0, mm, nn, p.alpha, p.gamma, p.re, p.dt, drag=p.drag, vdrag=p.vdrag,
+                      hypo=p.hypo, theta=p.theta, dragall=bool(p.dragall))
+    re, im = mw.ic_gaussian(p, 4, 0.2, 4.0)
+    v.set_state(re, im)
+    o.set_state(re, im)
+    v.vel_TS(20)
+    o.step(20)
+    err = rel_l2(v.get_state(), o.get_state())
+    e, eref = v.vel_energy(), o.energy()
+    print(f"plan {choice} k0={k0} mm={mm} nn={nn}: 20 steps rel_l2 {err:.2e}, energy {abs(e - eref) / eref:.2e}")
+    assert err <= 1e-12
+    assert abs(e - eref) <= 1e-12 * eref
+
+
+def test_timed_choice_is_cached(tuning):
+    tuning(1)
+    p = mw.make_params(6, 64, 512, 16 * math.pi, 128 * math.pi, 200.0, 0.02)
+    a = mw.Velocity(p)
+    b = mw.Velocity(p)
+    assert a.plan_info() in (0, 1, 2, 3)
+    assert b.plan_info() == a.plan_info()
+    # the tuner leaves a zero state and a forward-Euler first step
+    re, im = a.get_state()
+    assert not re.any() and not im.any()
+    assert a.get_onl()[2]
+    print(f"timed plan for M6 64x512: {a.plan_info()}")
