@@ -110,6 +110,11 @@ module mwf_capi
          real(c_double), intent(out) :: ans(*)
       end function mwf_vel_history
 
+      integer(c_int) function mwf_autotune(mode) bind(C, name='mwf_autotune')
+         import :: c_int, c_int32_t
+         integer(c_int32_t), value :: mode
+      end function mwf_autotune
+
       integer(c_int) function mwf_turb_sample(h, dm, dn, out) bind(C, name='mwf_turb_sample')
          import :: c_int, c_ptr, c_double, c_int32_t
          type(c_ptr), value :: h

@@ -62,6 +62,9 @@ contains
       p%hypo = d_HYPO
       p%theta = d_theta
       if (c_associated(mwf_h)) call mwf_check(mwf_destroy(mwf_h), 'mwf_destroy')
+      ! timed kernel plans, the part FFTW_PATIENT planning plays in
+      ! tra_precompute (program/transform.fftw5.f90:33-59)
+      call mwf_check(mwf_autotune(1_c_int32_t), 'mwf_autotune')
       call mwf_check(mwf_create(p, mwf_h), 'mwf_create')
       host_newer = .true.
       dev_newer = .false.

@@ -91,6 +91,10 @@ def test_fortran_driver_matches_oracle(tmp_path):
     w.set_state(re, im)
     w.vel_TS(nsteps)
     hre, him, first = w.get_onl()
-    assert np.array_equal(parts[5].reshape(NN, M, K), hre) and np.array_equal(parts[6].reshape(NN, M, K), him)
+    # the Fortran variant tunes its plan (mwf_autotune), the handle here uses the
+    # rule-based one: equal to rounding, not necessarily bit for bit
+    herr = math.sqrt((np.sum((parts[5].reshape(NN, M, K) - hre) ** 2) + np.sum((parts[6].reshape(NN, M, K) - him) ** 2))
+                     / (np.sum(hre ** 2) + np.sum(him ** 2)))
+    assert herr <= 1e-12, herr
     assert parts[7][0] == 0.0 and not first
     print("[fortran] device diagnostics and vel_onl match the Python handle")
