@@ -123,7 +123,7 @@ __device__ __forceinline__ void bfly_store(CT* __restrict__ db, int j, const CT*
     for (int r = 0; r < R; r++) db[pidx(o + r * Ns)] = x[r];
 }
 
-template <int L, int Ns, int RMAX = 8> constexpr int radix_at() { return (L / Ns) >= RMAX ? RMAX : (L / Ns); }
+template <int L, int Ns> constexpr int radix_at() { return (L / Ns) >= 8 ? 8 : (L / Ns); }
 
 template <int L, int Ns = 1> constexpr int npasses()
 {
@@ -158,12 +158,12 @@ template <int LOG> constexpr bool fft_in_b() { return (npasses<(1 << LOG)>() & 1
 
 // In-place FFT (all butterflies of a pass are loaded before any is stored);
 // for kernels whose LDS cannot hold two buffers.  SMAX bounds S.
-template <typename CT, int LOG, int SIGN, int T, int SMAX, int RMAX = 8, int Ns = 1>
+template <typename CT, int LOG, int SIGN, int T, int SMAX, int Ns = 1>
 __device__ __forceinline__ void ip_fft(CT* buf, int S, const CT* __restrict__ tw, int tid = threadIdx.x)
 {
     constexpr int L = 1 << LOG, LP = lpad(L);
     if constexpr (Ns < L) {
-        constexpr int R = radix_at<L, Ns, RMAX>();
+        constexpr int R = radix_at<L, Ns>();
         constexpr int NB = L / R;
         constexpr int MAXB = (SMAX * NB + T - 1) / T;
         const int total = S * NB;
@@ -180,7 +180,7 @@ __device__ __forceinline__ void ip_fft(CT* buf, int S, const CT* __restrict__ tw
             if (i < total) bfly_store<CT, R, Ns>(buf + (i / NB) * LP, i % NB, x[q]);
         }
         __syncthreads();
-        ip_fft<CT, LOG, SIGN, T, SMAX, RMAX, Ns * R>(buf, S, tw, tid);
+        ip_fft<CT, LOG, SIGN, T, SMAX, Ns * R>(buf, S, tw, tid);
     }
 }
 
