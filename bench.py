@@ -273,6 +273,8 @@ def main():
     ap.add_argument("--m6-err-steps", type=int, default=100)
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="rule-based kernel plans instead of timed ones")
+    ap.add_argument("--plan", type=int, default=None,
+                    help="force kernel plan bits (1 k_zb, 2 split x, 4 k_z2) instead of timing the plans")
     ap.add_argument("--cfg5-steps", type=int, default=200)
     ap.add_argument("--calib-copy", type=int, default=0, help="only run a MiB-sized device copy")
     args = ap.parse_args()
@@ -285,7 +287,9 @@ def main():
         sys.exit(2)
 
     import mwf_amd as mw
-    if not args.no_tune:
+    if args.plan is not None:
+        mw.autotune(2 + args.plan)
+    elif not args.no_tune:
         mw.autotune(1)  # timed plan per size class at handle creation (tra_precompute's role)
     p = mw.make_params(**WORKLOAD)
     ic = mw.ic_gaussian(p, IC["seed"], IC["scl"], IC["k0"])

@@ -130,12 +130,13 @@ int mwf_blend(mwf_handle *a, mwf_handle *b, double d, mwf_handle *dst);
  * handle or ensemble class of each (precision, k0, mm, nn) times its
  * candidate kernel plans at creation (a few steps each, on its zero state)
  * and keeps the fastest, and later ones of that size reuse the choice, so
- * results stay reproducible within a process; 2..5 = force plan choice
+ * results stay reproducible within a process; 2..9 = force plan choice
  * mode - 2 where that alternative exists (for tests and plan comparisons). */
 int mwf_autotune(int32_t mode);
 /* The plan a handle runs: bit 0 = phase-sequential z kernel instead of the
  * three-phase one, bit 1 = separate x-inverse and x-forward kernels instead
- * of the fused x kernel. */
+ * of the fused x kernel, bit 2 = three-phase z kernel with two points per
+ * thread (768-thread blocks) instead of three (512). */
 int mwf_plan_info(mwf_handle *h, int32_t *choice);
 
 /* Wait for all queued work of the handle. */

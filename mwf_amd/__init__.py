@@ -119,11 +119,11 @@ def _p(a):
 def autotune(mode=1):
     """Process-wide plan selection for handles created afterwards (mwf_autotune),
     the role of tra_precompute's FFTW_PATIENT planning: 0 = rule-based plans,
-    1 = timed choice per size, 2..5 = force plan bits mode - 2."""
+    1 = timed choice per size, 2..9 = force plan bits mode - 2."""
     _check(lib().mwf_autotune(int(mode)))
 
 
-PLAN_ZB, PLAN_SPLIT_X = 1, 2
+PLAN_ZB, PLAN_SPLIT_X, PLAN_Z2 = 1, 2, 4
 
 
 def dims(p):
@@ -238,7 +238,8 @@ class Velocity:
         return out
 
     def plan_info(self):
-        """Plan bits: PLAN_ZB (phase-sequential z kernel), PLAN_SPLIT_X (separate x kernels)."""
+        """Plan bits: PLAN_ZB (phase-sequential z kernel), PLAN_SPLIT_X (separate x kernels),
+        PLAN_Z2 (two-points-per-thread z kernel)."""
         ch = ctypes.c_int32()
         _check(lib().mwf_plan_info(self._h, ctypes.byref(ch)))
         return ch.value

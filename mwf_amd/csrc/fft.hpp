@@ -103,14 +103,95 @@ __device__ __forceinline__ void twiddle(CT* x, const CT* __restrict__ tw, int e)
     }
 }
 
+template <int L, int Ns> constexpr int radix_at() { return (L / Ns) >= 8 ? 8 : (L / Ns); }
+
+// ---------------------------------------------------------------------------
+// Pass twiddle tables.  In the pass with sub-transform length Ns, butterfly
+// j uses w^e, w^{2e}, w^{4e} with e = (j mod Ns) L/(Ns R).  Read from the
+// natural table, consecutive lanes of the Ns = 8 pass are 8 entries (128 B)
+// apart and all hit one bank group.  The pass table stores, for every pass
+// with Ns > 1 and every k < Ns, the triple (w^e, w^{2e}, w^{4e}) at
+// ptw_off<L, Ns>() + 3k (radix 2: w^e alone at ptw_off + k): lanes 48 (16) B
+// apart cover all eight 16-byte bank groups, the loads of one butterfly share
+// one address, and the values are those of the natural table (the third
+// entry of a radix-4 triple is filler).
+// ---------------------------------------------------------------------------
+template <int R> constexpr int ptw_width() { return R == 2 ? 1 : 3; }
+template <int L, int Ns = 1> constexpr int ptw_size()
+{
+    if constexpr (Ns >= L) return 0;
+    else return (Ns > 1 ? ptw_width<radix_at<L, Ns>()>() * Ns : 0) + ptw_size<L, Ns * radix_at<L, Ns>()>();
+}
+template <int L, int NsT, int Ns = 1> constexpr int ptw_off()
+{
+    if constexpr (Ns >= NsT) return 0;
+    else return (Ns > 1 ? ptw_width<radix_at<L, Ns>()>() * Ns : 0) + ptw_off<L, NsT, Ns * radix_at<L, Ns>()>();
+}
+
+// dst[i] for i = t, t + T, ... < ptw_size<L>() from the natural table tw (L entries).
+template <typename CT, int LOG, int Ns = 1>
+__device__ __forceinline__ void ptw_fill(CT* __restrict__ dst, const CT* __restrict__ tw, int t, int T)
+{
+    constexpr int L = 1 << LOG;
+    if constexpr (Ns < L) {
+        constexpr int R = radix_at<L, Ns>();
+        if constexpr (Ns > 1) {
+            constexpr int off = ptw_off<L, Ns>(), W = ptw_width<R>();
+            for (int i = t; i < W * Ns; i += T) {
+                const int k = i / W, m = i % W;
+                dst[off + i] = tw[((k * (L / (Ns * R))) << m) & (L - 1)];
+            }
+        }
+        ptw_fill<CT, LOG, Ns * R>(dst, tw, t, T);
+    }
+}
+
+// x[r] *= w^r from the pass table triple at p (conjugated for SIGN < 0)
+template <int R, int SIGN, typename CT>
+__device__ __forceinline__ void twiddle_p(CT* x, const CT* __restrict__ p)
+{
+    auto ld = [&](int i) {
+        CT w = p[i];
+        if (SIGN < 0) w = cconj(w);
+        return w;
+    };
+    if constexpr (R == 2) {
+        x[1] = cmul(x[1], ld(0));
+    } else if constexpr (R == 4) {
+        const CT w1 = ld(0), w2 = ld(1);
+        x[1] = cmul(x[1], w1);
+        x[2] = cmul(x[2], w2);
+        x[3] = cmul(x[3], cmul(w1, w2));
+    } else {
+        const CT w1 = ld(0), w2 = ld(1), w4 = ld(2);
+        const CT w3 = cmul(w1, w2);
+        x[1] = cmul(x[1], w1);
+        x[2] = cmul(x[2], w2);
+        x[3] = cmul(x[3], w3);
+        x[4] = cmul(x[4], w4);
+        x[5] = cmul(x[5], cmul(w1, w4));
+        x[6] = cmul(x[6], cmul(w2, w4));
+        x[7] = cmul(x[7], cmul(w3, w4));
+    }
+}
+
+// Twiddles of butterfly j in the pass (L, R, Ns): from the natural table, or
+// with PTW from the pass table.
+template <int L, int R, int Ns, int SIGN, bool PTW, typename CT>
+__device__ __forceinline__ void twiddle_j(CT* x, const CT* __restrict__ tw, int j)
+{
+    if constexpr (PTW) twiddle_p<R, SIGN>(x, tw + ptw_off<L, Ns>() + ptw_width<R>() * (j & (Ns - 1)));
+    else twiddle<R, SIGN>(x, tw, (j & (Ns - 1)) * (L / (Ns * R)));
+}
+
 // One radix-R butterfly j of sequence base (load, twiddle, DFT); outputs stay in x.
-template <typename CT, int L, int R, int Ns, int SIGN>
+template <typename CT, int L, int R, int Ns, int SIGN, bool PTW = false>
 __device__ __forceinline__ void bfly_load(const CT* __restrict__ sb, int j, CT* x, const CT* __restrict__ tw)
 {
     constexpr int NB = L / R;
 #pragma unroll
     for (int r = 0; r < R; r++) x[r] = sb[pidx(j + r * NB)];
-    if constexpr (Ns > 1) twiddle<R, SIGN>(x, tw, (j & (Ns - 1)) * (L / (Ns * R)));
+    if constexpr (Ns > 1) twiddle_j<L, R, Ns, SIGN, PTW>(x, tw, j);
     dftR<R, SIGN>(x);
 }
 
@@ -123,8 +204,6 @@ __device__ __forceinline__ void bfly_store(CT* __restrict__ db, int j, const CT*
     for (int r = 0; r < R; r++) db[pidx(o + r * Ns)] = x[r];
 }
 
-template <int L, int Ns> constexpr int radix_at() { return (L / Ns) >= 8 ? 8 : (L / Ns); }
-
 template <int L, int Ns = 1> constexpr int npasses()
 {
     if constexpr (Ns >= L) return 0;
@@ -134,7 +213,7 @@ template <int L, int Ns = 1> constexpr int npasses()
 // Ping-pong FFT over S sequences: pass 0 reads A and writes B, pass 1 reads B
 // and writes A, ...  Ends with a barrier.  The result is in B when the pass
 // count is odd (fft_in_b<LOG>()), else in A.
-template <typename CT, int LOG, int SIGN, int T, int Ns = 1, int PASS = 0>
+template <typename CT, int LOG, int SIGN, int T, bool PTW = false, int Ns = 1, int PASS = 0>
 __device__ __forceinline__ void pp_fft(CT* A, CT* B, int S, const CT* __restrict__ tw)
 {
     constexpr int L = 1 << LOG, LP = lpad(L);
@@ -147,18 +226,18 @@ __device__ __forceinline__ void pp_fft(CT* A, CT* B, int S, const CT* __restrict
         for (int i = threadIdx.x; i < total; i += T) {
             const int s = i / NB, j = i % NB;
             CT x[R];
-            bfly_load<CT, L, R, Ns, SIGN>(src + s * LP, j, x, tw);
+            bfly_load<CT, L, R, Ns, SIGN, PTW>(src + s * LP, j, x, tw);
             bfly_store<CT, R, Ns>(dst + s * LP, j, x);
         }
         __syncthreads();
-        pp_fft<CT, LOG, SIGN, T, Ns * R, PASS + 1>(A, B, S, tw);
+        pp_fft<CT, LOG, SIGN, T, PTW, Ns * R, PASS + 1>(A, B, S, tw);
     }
 }
 template <int LOG> constexpr bool fft_in_b() { return (npasses<(1 << LOG)>() & 1) != 0; }
 
 // In-place FFT (all butterflies of a pass are loaded before any is stored);
 // for kernels whose LDS cannot hold two buffers.  SMAX bounds S.
-template <typename CT, int LOG, int SIGN, int T, int SMAX, int Ns = 1>
+template <typename CT, int LOG, int SIGN, int T, int SMAX, bool PTW = false, int Ns = 1>
 __device__ __forceinline__ void ip_fft(CT* buf, int S, const CT* __restrict__ tw, int tid = threadIdx.x)
 {
     constexpr int L = 1 << LOG, LP = lpad(L);
@@ -171,7 +250,7 @@ __device__ __forceinline__ void ip_fft(CT* buf, int S, const CT* __restrict__ tw
 #pragma unroll
         for (int q = 0; q < MAXB; q++) {
             const int i = tid + q * T;
-            if (i < total) bfly_load<CT, L, R, Ns, SIGN>(buf + (i / NB) * LP, i % NB, x[q], tw);
+            if (i < total) bfly_load<CT, L, R, Ns, SIGN, PTW>(buf + (i / NB) * LP, i % NB, x[q], tw);
         }
         __syncthreads();
 #pragma unroll
@@ -180,7 +259,7 @@ __device__ __forceinline__ void ip_fft(CT* buf, int S, const CT* __restrict__ tw
             if (i < total) bfly_store<CT, R, Ns>(buf + (i / NB) * LP, i % NB, x[q]);
         }
         __syncthreads();
-        ip_fft<CT, LOG, SIGN, T, SMAX, Ns * R>(buf, S, tw, tid);
+        ip_fft<CT, LOG, SIGN, T, SMAX, PTW, Ns * R>(buf, S, tw, tid);
     }
 }
 
@@ -200,7 +279,7 @@ __device__ __forceinline__ void wave_lds_sync()
 
 template <int L> constexpr int group_lanes() { return L / 8 < 64 ? L / 8 : 64; }
 
-template <typename CT, int LOG, int SIGN, int Ns = 1, int PASS = 0>
+template <typename CT, int LOG, int SIGN, bool PTW = false, int Ns = 1, int PASS = 0>
 __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, const CT* __restrict__ tw)
 {
     constexpr int L = 1 << LOG;
@@ -220,7 +299,7 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
 #pragma unroll
         for (int i = 0; i < NBL; i++) {
             const int j = gl + G * i;
-            if constexpr (Ns > 1) twiddle<R, SIGN>(x[i], tw, (j & (Ns - 1)) * (L / (Ns * R)));
+            if constexpr (Ns > 1) twiddle_j<L, R, Ns, SIGN, PTW>(x[i], tw, j);
             dftR<R, SIGN>(x[i]);
         }
         if constexpr (LAST) {
@@ -239,7 +318,7 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
                 for (int r = 0; r < R; r++) reg[pidx(o + r * Ns)] = x[i][r];
             }
             wave_lds_sync();
-            group_fft<CT, LOG, SIGN, Ns * R, PASS + 1>(v, reg, gl, tw);
+            group_fft<CT, LOG, SIGN, PTW, Ns * R, PASS + 1>(v, reg, gl, tw);
         }
     }
 }

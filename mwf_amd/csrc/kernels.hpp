@@ -315,10 +315,10 @@ __device__ __forceinline__ cplx_t<R> zrow(const Geo& g, const cplx_t<R>* __restr
 // fr < NN/2 writes Z[fr] and Z[NN-fr] from a = X[fr], m = X[NN-fr] of both rows
 // (fr = 0: the DC bin with its imaginary part dropped, and the self-mirrored
 // NN/2 bin, Y = 2 Re(X w^{(NN/2) b})).
+// wr = w^{fr b} and wm = w^{(NN-fr) b} (fr = 0: wm = w^{(NN/2) b}); unused for b = 0.
 template <typename R>
-__device__ __forceinline__ void zfold_b(cplx_t<R>* __restrict__ seq, int b, int fr, int NN, cplx_t<R> a1,
-                                        cplx_t<R> b1, cplx_t<R> a2, cplx_t<R> b2,
-                                        const cplx_t<R>* __restrict__ twQ)
+__device__ __forceinline__ void zfold_bw(cplx_t<R>* __restrict__ seq, int b, int fr, int NN, cplx_t<R> a1,
+                                         cplx_t<R> b1, cplx_t<R> a2, cplx_t<R> b2, cplx_t<R> wr, cplx_t<R> wm)
 {
     using CT = cplx_t<R>;
     const int H = NN / 2;
@@ -327,8 +327,7 @@ __device__ __forceinline__ void zfold_b(cplx_t<R>* __restrict__ seq, int b, int 
         if (b == 0) {
             seq[pidx(H)] = CT{(R)2 * b1.x, (R)2 * b2.x};
         } else {
-            const CT w = twQ[H * b];
-            const CT p1 = cmul(b1, w), p2 = cmul(b2, w);
+            const CT p1 = cmul(b1, wm), p2 = cmul(b2, wm);
             seq[pidx(H)] = CT{(R)2 * p1.x, (R)2 * p2.x};
         }
         return;
@@ -336,7 +335,7 @@ __device__ __forceinline__ void zfold_b(cplx_t<R>* __restrict__ seq, int b, int 
     const int fm = NN - fr;
     CT P1a = a1, P2a = b1, P1b = a2, P2b = b2; // X[s] w^{sb}, X[NN-s] w^{(NN-s)b}
     if (b > 0) {
-        const CT w1 = twQ[fr * b], w2 = twQ[fm * b];
+        const CT w1 = wr, w2 = wm;
         P1a = cmul(a1, w1);
         P2a = cmul(b1, w2);
         P1b = cmul(a2, w1);
@@ -349,14 +348,40 @@ __device__ __forceinline__ void zfold_b(cplx_t<R>* __restrict__ seq, int b, int 
     seq[pidx(fm)] = CT{y1m.x - y2m.y, y1m.y + y2m.x};
 }
 
-// two transform buffers, then copies of the twiddle tables: NN entries of
-// exp(2 pi i j/NN) and the first 2 NN entries of exp(2 pi i j/Q) (fold and
-// r2c unpacking use indices below 2 NN).  Twiddles read from LDS instead of
-// global memory keep their latency off every transform pass.
+template <typename R>
+__device__ __forceinline__ void zfold_b(cplx_t<R>* __restrict__ seq, int b, int fr, int NN, cplx_t<R> a1,
+                                        cplx_t<R> b1, cplx_t<R> a2, cplx_t<R> b2,
+                                        const cplx_t<R>* __restrict__ twQ)
+{
+    using CT = cplx_t<R>;
+    CT wr{(R)1, (R)0}, wm{(R)1, (R)0};
+    if (b > 0) {
+        wr = twQ[fr * b];
+        wm = twQ[(fr == 0 ? NN / 2 : NN - fr) * b];
+    }
+    zfold_bw<R>(seq, b, fr, NN, a1, b1, a2, b2, wr, wm);
+}
+
+// w^k = exp(2 pi i k/Q), Q = 3 NN, for 0 <= k < 2 NN from a table of the
+// first NN entries: w^{NN + j} = exp(2 pi i/3) w^j.
+template <typename R>
+__device__ __forceinline__ cplx_t<R> twq2(const cplx_t<R>* __restrict__ tw, int NN, int k)
+{
+    using CT = cplx_t<R>;
+    if (k < NN) return tw[k];
+    const CT w = tw[k - NN];
+    const R c = (R)-0.5, sn = (R)0.86602540378443864676; // exp(2 pi i/3)
+    return CT{c * w.x - sn * w.y, c * w.y + sn * w.x};
+}
+
+// two transform buffers, then the pass twiddle table of the NN-point
+// transforms (ptw_size entries) and the first 2 NN entries of
+// exp(2 pi i j/Q) (fold and r2c unpacking use indices below 2 NN).  Twiddles
+// read from LDS instead of global memory keep their latency off every pass.
 template <typename R, int K0, int LOGNN> constexpr size_t z_lds_bytes()
 {
     constexpr int NN = 1 << LOGNN, XB = ZT / NN, ZP = zpairs<R>();
-    return sizeof(cplx_t<R>) * (2 * (size_t)XB * ZP * 3 * lpad(NN) + 3 * (size_t)NN);
+    return sizeof(cplx_t<R>) * (2 * (size_t)XB * ZP * 3 * lpad(NN) + ptw_size<NN>() + 2 * (size_t)NN);
 }
 
 template <typename R, int K0, int LOGNN>
@@ -389,8 +414,8 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     extern __shared__ unsigned char smem_raw[];
     CT* A = reinterpret_cast<CT*>(smem_raw);        // [SEQ][LP]
     CT* B = A + SEQ * LP;                           // [SEQ][LP]
-    CT* twNl = B + SEQ * LP;                        // [NN]
-    CT* twQl = twNl + NN;                           // [2 NN]
+    CT* twNl = B + SEQ * LP;                        // [ptw_size<NN>()] pass twiddles
+    CT* twQl = twNl + ptw_size<NN>();               // [2 NN]
     // Group form: each sequence of a batch is transformed by its own group of
     // GL lanes (group_fft: 8 points per lane, wave-local exchanges), so the
     // waves run their transforms without block barriers and overlap each
@@ -403,7 +428,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
             CT v[NN / GL];
 #pragma unroll
             for (int q = 0; q < NN / GL; q++) v[q] = sr[pidx(gl + GL * q)];
-            group_fft<CT, LOGNN, decltype(sign)::value>(v, sr, gl, twNl);
+            group_fft<CT, LOGNN, decltype(sign)::value, true>(v, sr, gl, twNl);
 #pragma unroll
             for (int q = 0; q < NN / GL; q++) sr[pidx(gl + GL * q)] = v[q];
         }
@@ -452,7 +477,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
         for (int s = 0; s < KK; s++) an[b][s] = (R)0;
 
     prefetch(0);
-    for (int i = tid; i < NN; i += ZT) twNl[i] = twNN[i];
+    ptw_fill<CT, LOGNN>(twNl, twNN, tid, ZT);
     for (int i = tid; i < 2 * NN; i += ZT) twQl[i] = twQ[i];
     __syncthreads();
     static_for<0, NBI>([&](auto bi) {
@@ -476,7 +501,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
             seq_fft(F, std::integral_constant<int, +1>{});
             __syncthreads();
         } else {
-            pp_fft<CT, LOGNN, +1, ZT>(A, B, SEQ, twNl);
+            pp_fft<CT, LOGNN, +1, ZT, true>(A, B, SEQ, twNl);
             F = INB ? B : A;
         }
         // consume: thread (xl, a = r)
@@ -522,7 +547,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
             seq_fft(F, std::integral_constant<int, -1>{});
             __syncthreads();
         } else {
-            pp_fft<CT, LOGNN, -1, ZT>(A, B, SEQ, twNl);
+            pp_fft<CT, LOGNN, -1, ZT, true>(A, B, SEQ, twNl);
             F = INB ? B : A;
         }
         static_for<0, ZP>([&](auto pi) {
@@ -557,6 +582,202 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// z fused with two points per thread.  A block of Z2T = 768 threads covers
+// XB = 512/NN x-rows, 1536 physical points; thread tid owns the flat points
+// f = tid and tid + Z2T, f = (xl*3 + b)*NN + a (row xl, z = 3a + b), so each
+// thread keeps 2 (not 3) points of base values and accumulators and the
+// block runs 3 waves per SIMD instead of 2.  Batches hold Z2P = 3 pairs
+// (9 sequences per x-row, 576 radix-8 butterflies per pass: most threads
+// busy).  The fold, transforms, products and r2c unpacking are those of k_z;
+// the passes ping-pong between two buffers.  The fold and unpack twiddles
+// w^k, k < 2 NN, come from an LDS copy of the first NN entries of the Q
+// table (twq2), which keeps the block within 160 KB at NN = 512 in FP64.
+// ---------------------------------------------------------------------------
+constexpr int Z2T = 768;
+constexpr int Z2P = 3;
+
+template <typename R, int LOGNN> constexpr size_t z2_lds_bytes()
+{
+    constexpr int NN = 1 << LOGNN, XB = 512 / NN;
+    constexpr size_t seq = (size_t)XB * Z2P * 3 * lpad(NN);
+    return sizeof(cplx_t<R>) * (2 * seq + ptw_size<NN>() + (size_t)NN);
+}
+
+template <typename R, int K0, int LOGNN>
+__global__ __launch_bounds__(Z2T) void k_z2(Batch bt, const cplx_t<R>* __restrict__ U, cplx_t<R>* __restrict__ NL,
+                                            const cplx_t<R>* __restrict__ twNN, const cplx_t<R>* __restrict__ twQ)
+{
+    using CT = cplx_t<R>;
+    const Geo g = bt.geo[blockIdx.z];
+    U += blockIdx.z * bt.u_stride;
+    NL += blockIdx.z * bt.nl_stride;
+    using G = Gal<K0>;
+    constexpr int KK = 3 * K0 - 1;
+    constexpr int NN = 1 << LOGNN;
+    constexpr int XB = 512 / NN;
+    using ZO = ZOrder<K0>;
+    constexpr int ZP = Z2P;
+    constexpr int NBI = (ZO::NP + ZP - 1) / ZP;
+    constexpr int NPO = (KK + 1) / 2, NBO = (NPO + ZP - 1) / ZP;
+    constexpr int SEQ = XB * ZP * 3;
+    constexpr int LP = lpad(NN);
+    constexpr int H = NN / 2;
+    constexpr int PT = 3 * XB * NN / Z2T;      // points per thread
+    constexpr int UT = XB * ZP * NN / Z2T;     // r2c unpack tasks per thread
+    constexpr bool INB = fft_in_b<LOGNN>();
+    static_assert(XB * ZP * H == Z2T, "one fold task per thread");
+    static_assert(PT * Z2T == 3 * XB * NN && UT * Z2T == XB * ZP * NN, "points and tasks tile the block");
+    extern __shared__ unsigned char smem_raw[];
+    CT* A = reinterpret_cast<CT*>(smem_raw);   // [SEQ][LP]
+    CT* B = A + SEQ * LP;                      // [SEQ][LP]
+    CT* twNl = B + SEQ * LP;                   // [ptw_size<NN>()] pass twiddles
+    CT* twQl = twNl + ptw_size<NN>();          // [NN]: w^k, k < NN
+
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * XB;
+    const bool wide = g.NTx * (int)sizeof(CT) >= 64;
+    // Thread-derived indices are recomputed per batch from an opaque copy of
+    // the thread index, so that the compiler does not hoist every batch's
+    // addresses into registers next to the point state.
+    auto opaque_tid = [] {
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        return t;
+    };
+    // owned point j: LDS offset of (xl, pair 0, b, a); pair p adds 3 p LP
+    auto poff = [](int t, int j) {
+        const int f = t + Z2T * j;
+        const int xl = f / (3 * NN), b = (f / NN) % 3, a = f % NN;
+        return (xl * ZP * 3 + b) * LP + pidx(a);
+    };
+
+    // fold task of thread t: (row fx, pair fp, index fr)
+    CT pf[4];
+    auto prefetch = [&](int t, int batch) {
+        const int fx = t / (ZP * H), fp = (t / H) % ZP, fr = t % H;
+        const int xg = x0 + fx;
+        const int pr = batch * ZP + fp;
+        const int r2 = fr == 0 ? H : NN - fr;
+        if (xg < g.P) {
+            const int ka = ZO::kind(2 * pr), sa = ZO::slot(2 * pr);
+            pf[0] = zrow<R, K0>(g, U, xg, ka, sa, fr);
+            pf[1] = zrow<R, K0>(g, U, xg, ka, sa, r2);
+            if (ZO::shared(pr)) {
+                pf[2] = zderiv<R>(g, pf[0], fr);
+                pf[3] = zderiv<R>(g, pf[1], r2);
+            } else {
+                const int kb = ZO::kind(2 * pr + 1), sb = ZO::slot(2 * pr + 1);
+                pf[2] = zrow<R, K0>(g, U, xg, kb, sb, fr);
+                pf[3] = zrow<R, K0>(g, U, xg, kb, sb, r2);
+            }
+        } else {
+            pf[0] = pf[1] = pf[2] = pf[3] = CT{(R)0, (R)0};
+        }
+    };
+    auto transform = [&](auto sign) { pp_fft<CT, LOGNN, decltype(sign)::value, Z2T, true>(A, B, SEQ, twNl); };
+
+    R vb[PT][KK];
+    R an[PT][KK];
+#pragma unroll
+    for (int j = 0; j < PT; j++)
+#pragma unroll
+        for (int s = 0; s < KK; s++) an[j][s] = (R)0;
+
+    prefetch(threadIdx.x, 0);
+    ptw_fill<CT, LOGNN>(twNl, twNN, threadIdx.x, Z2T);
+    for (int i = threadIdx.x; i < NN; i += Z2T) twQl[i] = twQ[i];
+    __syncthreads();
+    CT* const F = INB ? B : A; // where a transform's result lies
+    static_for<0, NBI>([&](auto bi) {
+        constexpr int BN = decltype(bi)::value;
+        const int tid = opaque_tid();
+        {
+            const int fx = tid / (ZP * H), fp = (tid / H) % ZP, fr = tid % H;
+            const int fm = fr == 0 ? H : NN - fr;
+            CT* seq = A + ((fx * ZP + fp) * 3) * LP;
+            const CT one{(R)1, (R)0};
+            zfold_bw<R>(seq, 0, fr, NN, pf[0], pf[1], pf[2], pf[3], one, one);
+            zfold_bw<R>(seq + LP, 1, fr, NN, pf[0], pf[1], pf[2], pf[3], twQl[fr], twQl[fm]);
+            zfold_bw<R>(seq + 2 * LP, 2, fr, NN, pf[0], pf[1], pf[2], pf[3], twQl[2 * fr],
+                        twq2<R>(twQl, NN, 2 * fm));
+        }
+        if constexpr (BN + 1 < NBI) prefetch(tid, BN + 1);
+        __syncthreads();
+        transform(std::integral_constant<int, +1>{});
+        static_for<0, ZP>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+#pragma unroll
+            for (int j = 0; j < PT; j++) {
+                const CT zv = F[poff(tid, j) + p * 3 * LP];
+                static_for<0, 2>([&](auto hi) {
+                    constexpr int q = BN * 2 * ZP + 2 * p + decltype(hi)::value;
+                    constexpr int kd = ZO::kind(q), s = ZO::slot(q);
+                    const R val = decltype(hi)::value == 0 ? zv.x : zv.y;
+                    if constexpr (kd == 0) vb[j][s] = val;
+                    else if constexpr (kd == 1) G::template dx<s>(vb[j], val, an[j]);
+                    else if constexpr (kd == 2) G::template dz<s>(vb[j], val, an[j]);
+                });
+            }
+        });
+        if constexpr (!INB) __syncthreads();
+    });
+#pragma unroll
+    for (int j = 0; j < PT; j++) G::yterms(vb[j], an[j], (R)g.beta);
+
+    // forward z transform of the KK products, two per complex sequence
+    const R scale = (R)g.scale;
+    static_for<0, NBO>([&](auto ri) {
+        constexpr int RD = decltype(ri)::value;
+        const int tid = opaque_tid();
+        static_for<0, ZP>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+            constexpr int c1 = 2 * (RD * ZP + p), c2 = c1 + 1;
+            if constexpr (c1 < KK) {
+#pragma unroll
+                for (int j = 0; j < PT; j++) {
+                    R v2 = (R)0;
+                    if constexpr (c2 < KK) v2 = an[j][c2];
+                    A[poff(tid, j) + p * 3 * LP] = CT{an[j][c1], v2};
+                }
+            }
+        });
+        __syncthreads();
+        transform(std::integral_constant<int, -1>{});
+#pragma unroll
+        for (int u = 0; u < UT; u++) {
+            const int t = tid + Z2T * u;
+            const int r = t % NN, p = (t / NN) % ZP, xl = t / (NN * ZP);
+            const int c1 = 2 * (RD * ZP + p), c2 = c1 + 1;
+            if (c1 < KK) {
+                const CT* fs = F + (xl * ZP + p) * 3 * LP;
+                const int rm = (NN - r) & (NN - 1);
+                CT X1{(R)0, (R)0}, X2{(R)0, (R)0};
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    const CT zn = fs[b * LP + pidx(r)];
+                    const CT zm = fs[b * LP + pidx(rm)];
+                    const CT F1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
+                    const CT F2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
+                    if (b == 0) {
+                        X1 = F1;
+                        X2 = F2;
+                    } else {
+                        const CT w = cconj(twq2<R>(twQl, NN, r * b));
+                        X1 = cadd(X1, cmul(F1, w));
+                        X2 = cadd(X2, cmul(F2, w));
+                    }
+                }
+                const int x = x0 + xl;
+                if (x < g.P) {
+                    st_piece(&NL[nidx(g, x, c1, r)], cscale(X1, scale), wide);
+                    if (c2 < KK) st_piece(&NL[nidx(g, x, c2, r)], cscale(X2, scale), wide);
+                }
+            }
+        }
+        if constexpr (!INB) __syncthreads();
+    });
+}
+
+// ---------------------------------------------------------------------------
 // z pass with the three fold phases in sequence (b = 0, 1, 2), for rows whose
 // per-point state does not fit the registers of one block when all three
 // phases are live (NN = 1024, and the 17-slot M6 basis).  A block of ZB
@@ -578,9 +799,9 @@ template <int NP, int SMAX> constexpr int zb_split() { return (NP + SMAX - 1) / 
 template <int NP, int SMAX> constexpr int zb_per() { return (NP + zb_split<NP, SMAX>() - 1) / zb_split<NP, SMAX>(); }
 // transform buffer, then the twiddle copies (NN of exp(2 pi i j/NN), the first
 // 2 NN of exp(2 pi i j/Q)) for NN up to ZB
-template <typename R> constexpr size_t zb_lds_bytes()
+template <typename R, int LOGNN> constexpr size_t zb_lds_bytes()
 {
-    return sizeof(cplx_t<R>) * ((size_t)zb_seq<R>() * ZB + (zb_twl<R>() ? 3 * (size_t)ZB : 0));
+    return sizeof(cplx_t<R>) * ((size_t)zb_seq<R>() * ZB + ptw_size<(1 << LOGNN)>() + (zb_twl<R>() ? 2 * ((size_t)1 << LOGNN) : 0));
 }
 
 template <typename R, int K0, int LOGNN>
@@ -607,14 +828,13 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
     constexpr int NBO = zb_split<NPO, SMAX>();
     extern __shared__ unsigned char smem_raw[];
     CT* A = reinterpret_cast<CT*>(smem_raw);                  // [XB*ZP][LP]
-    CT* twNl = A + zb_seq<R>() * ZB;                          // [NN]
-    CT* twQl = twNl + NN;                                     // [2 NN]
-    if constexpr (zb_twl<R>()) {
-        for (int i = threadIdx.x; i < NN; i += ZB) twNl[i] = twNN[i];
+    CT* twNl = A + zb_seq<R>() * ZB;                          // [ptw_size<NN>()] pass twiddles
+    CT* twQl = twNl + ptw_size<NN>();                         // [2 NN] (when zb_twl)
+    ptw_fill<CT, LOGNN>(twNl, twNN, threadIdx.x, ZB);
+    if constexpr (zb_twl<R>())
         for (int i = threadIdx.x; i < 2 * NN; i += ZB) twQl[i] = twQ[i];
-        __syncthreads();
-    }
-    const CT* tN = zb_twl<R>() ? twNl : twNN;
+    __syncthreads();
+    const CT* tN = twNl;
     const CT* tQ = zb_twl<R>() ? twQl : twQ;
 
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * XB;
@@ -657,7 +877,7 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
                 zfold_b<R>(A + (fx * ZP + fp) * LP, b, fr, NN, a1, b1, a2, b2, tQ);
             }
             __syncthreads();
-            ip_fft<CT, LOGNN, +1, ZB, XB * ZP>(A, XB * ZP, tN, tid);
+            ip_fft<CT, LOGNN, +1, ZB, XB * ZP, true>(A, XB * ZP, tN, tid);
             static_for<0, ZP>([&](auto pi) {
                 constexpr int p = decltype(pi)::value;
                 const CT zv = A[(xl * ZP + p) * LP + pidx(r)];
@@ -706,7 +926,7 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
                 }
             });
             __syncthreads();
-            ip_fft<CT, LOGNN, -1, ZB, XB * ZO_>(A, XB * ZO_, tN, tid);
+            ip_fft<CT, LOGNN, -1, ZB, XB * ZO_, true>(A, XB * ZO_, tN, tid);
             static_for<0, ZO_>([&](auto pi) {
                 constexpr int p = decltype(pi)::value;
                 constexpr int c1 = 2 * (RD * ZO_ + p), c2 = c1 + 1;
@@ -1043,8 +1263,9 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     constexpr int LPT = 3 * XTILE / XS;        // rows per thread in the cooperative load/store
     extern __shared__ unsigned char smem_raw[];
     CT* reg = reinterpret_cast<CT*>(smem_raw); // [XS/G sequences][MM], swizzled
-    CT* twMl = reg + (XS / G) * MM;            // exp(2 pi i j/MM), j < MM
-    double2* twPl = reinterpret_cast<double2*>(twMl + MM); // exp(2 pi i j/P), j <= 2 MM
+    CT* twMl = reg + (XS / G) * MM;            // pass twiddles of the MM-point transforms
+    static_assert(ptw_size<MM>() % 2 == 0, "twPl must stay 16-byte aligned");
+    double2* twPl = reinterpret_cast<double2*>(twMl + ptw_size<MM>()); // exp(2 pi i j/P), j <= 2 MM
 
     const Geo g = bt.geo[blockIdx.z];
     NL += blockIdx.z * bt.nl_stride;
@@ -1068,7 +1289,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     // update both hold the new state
     double2 psi[IPT][2], phi[IPT][2];
 
-    for (int i = tid; i < MM; i += XS) twMl[i] = twMM[i];
+    ptw_fill<CT, LOGMM>(twMl, twMM, tid, XS);
     for (int i = tid; i <= 2 * MM; i += XS) twPl[i] = twP[i];
 
     if constexpr (!INV_ONLY) {
@@ -1094,7 +1315,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                 CT v[PL];
 #pragma unroll
                 for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
-                group_fft<CT, LOGMM, -1>(v, myreg, gl, twMl);
+                group_fft<CT, LOGMM, -1, true>(v, myreg, gl, twMl);
 #pragma unroll
                 for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
             }
@@ -1229,7 +1450,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             CT v[PL];
 #pragma unroll
             for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
-            group_fft<CT, LOGMM, +1>(v, myreg, gl, twMl);
+            group_fft<CT, LOGMM, +1, true>(v, myreg, gl, twMl);
 #pragma unroll
             for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
         }

@@ -60,9 +60,12 @@ struct Plan {
     // fused forward + update + next inverse
     hipError_t (*xstep)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
                         const void*, const double2*, int, int) = nullptr;
-    // alternative z kernel the autotuner may pick (k_zb where k_z also runs)
+    // alternative z kernels the autotuner may pick: k_zb where k_z also runs,
+    // and the two-points-per-thread k_z2
     hipError_t (*zf_alt)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*,
                          const void*) = nullptr;
+    hipError_t (*zf_alt2)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*,
+                          const void*) = nullptr;
 };
 
 template <typename F> static void set_lds(F* f, size_t bytes)
@@ -99,11 +102,13 @@ static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream
     return hipGetLastError();
 }
 
-// transform buffers, then the MM-entry x twiddles and 2 MM + 1 fold twiddles
+// transform buffers, then the pass twiddles of the MM-point transforms and
+// 2 MM + 1 fold twiddles
 template <typename R, int LOGMM> static size_t xstep_lds()
 {
     constexpr size_t MM = (size_t)1 << LOGMM;
-    return (xs_of<LOGMM>() / group_lanes<(1 << LOGMM)>() + 1) * MM * sizeof(cplx_t<R>) + (2 * MM + 1) * sizeof(double2);
+    return ((xs_of<LOGMM>() / group_lanes<(1 << LOGMM)>()) * MM + ptw_size<(1 << LOGMM)>()) * sizeof(cplx_t<R>) +
+           (2 * MM + 1) * sizeof(double2);
 }
 
 template <typename R, int LOGMM>
@@ -134,13 +139,25 @@ static hipError_t launch_z(const Batch& bt, int nmem, const Geo& g, hipStream_t 
 }
 
 template <typename R, int K0, int LOGNN>
+static hipError_t launch_z2(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
+                            const void* twNN, const void* twQ)
+{
+    using CT = cplx_t<R>;
+    constexpr int XB = 512 >> LOGNN;
+    dim3 grid((g.P + XB - 1) / XB, 1, nmem);
+    hipLaunchKernelGGL((k_z2<R, K0, LOGNN>), grid, dim3(Z2T), (z2_lds_bytes<R, LOGNN>()), s, bt,
+                       (const CT*)U, (CT*)NL, (const CT*)twNN, (const CT*)twQ);
+    return hipGetLastError();
+}
+
+template <typename R, int K0, int LOGNN>
 static hipError_t launch_zb(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
                             const void* twNN, const void* twQ)
 {
     using CT = cplx_t<R>;
     constexpr int XB = ZB >> LOGNN;
     dim3 grid((g.P + XB - 1) / XB, 1, nmem);
-    hipLaunchKernelGGL((k_zb<R, K0, LOGNN>), grid, dim3(ZB), (zb_lds_bytes<R>()), s, bt, (const CT*)U, (CT*)NL,
+    hipLaunchKernelGGL((k_zb<R, K0, LOGNN>), grid, dim3(ZB), (zb_lds_bytes<R, LOGNN>()), s, bt, (const CT*)U, (CT*)NL,
                        (const CT*)twNN, (const CT*)twQ);
     return hipGetLastError();
 }
@@ -188,6 +205,7 @@ template <typename R, int K0> static void pick_z(int ln, Plan& p)
 #define C_(L)                                   \
     case L:                                     \
         p.zf = launch_z<R, K0, L>;              \
+        p.zf_alt2 = launch_z2<R, K0, L>;        \
         p.zphys = launch_zphys<R, K0, L>;       \
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
@@ -235,6 +253,7 @@ template <typename R, int K0> static void prepare_z(int ln)
 #define C_(L)                                                     \
     case L:                                                       \
         set_lds(k_z<R, K0, L>, z_lds_bytes<R, K0, L>());          \
+        set_lds(k_z2<R, K0, L>, z2_lds_bytes<R, L>());            \
         set_lds(k_zphys<R, K0, L>, zphys_lds<R, L>());            \
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
@@ -242,10 +261,10 @@ template <typename R, int K0> static void prepare_z(int ln)
     default: break;
     }
     switch (ln) {
-    case 8: set_lds(k_zb<R, K0, 8>, zb_lds_bytes<R>()); break;
-    case 9: set_lds(k_zb<R, K0, 9>, zb_lds_bytes<R>()); break;
+    case 8: set_lds(k_zb<R, K0, 8>, zb_lds_bytes<R, 8>()); break;
+    case 9: set_lds(k_zb<R, K0, 9>, zb_lds_bytes<R, 9>()); break;
     case 10:
-        set_lds(k_zb<R, K0, 10>, zb_lds_bytes<R>());
+        set_lds(k_zb<R, K0, 10>, zb_lds_bytes<R, 10>());
         set_lds(k_zphys<R, K0, 10>, zphys_lds<R, 10>());
         break;
     default: break;
@@ -351,12 +370,13 @@ template <typename T> static int dev_upload(T** d, const T* h, size_t n)
 // (program/transform.fftw5.f90:33-59).  Mode 0 (default): the rule-based plan
 // (k_z for NN <= 512, the fused x kernel wherever it exists).  Mode 1: each
 // new size class times its candidate plans on its own buffers (zero state,
-// 2 + 3 steps each) and keeps the fastest; the choice is cached per
-// (precision, K0, MM, NN) for the process, so every handle and ensemble class
-// of one size runs the same kernels.  Modes 2..5 force choice mode - 2 (where
-// the alternative exists), for tests and plan comparisons.  Choice bits: 1 =
-// phase-sequential z kernel (k_zb) instead of k_z, 2 = x-inverse + x-forward
-// instead of the fused x kernel.
+// best of 3 interleaved rounds of 5 steps) and keeps the fastest; the choice
+// is cached per (precision, K0, MM, NN) for the process, so every handle and
+// ensemble class of one size runs the same kernels.  Modes 2..9 force choice
+// mode - 2 (where the alternative exists), for tests and plan comparisons.
+// Choice bits: 1 = phase-sequential z kernel (k_zb) instead of k_z, 2 =
+// x-inverse + x-forward instead of the fused x kernel, 4 = two-points-per-
+// thread z kernel (k_z2; takes precedence over bit 1).
 // ---------------------------------------------------------------------------
 static std::atomic<int> g_tune{0};
 static std::mutex g_tune_mu;
@@ -436,7 +456,8 @@ struct SizeClass {
     {
         choice = ch;
         plan = base;
-        if ((ch & 1) && base.zf_alt) plan.zf = base.zf_alt;
+        if ((ch & 4) && base.zf_alt2) plan.zf = base.zf_alt2;
+        else if ((ch & 1) && base.zf_alt) plan.zf = base.zf_alt;
         if (ch & 2) plan.xstep = nullptr;
     }
 
@@ -460,7 +481,10 @@ struct SizeClass {
             return 0;
         }
         std::vector<int> cand;
-        for (int z = 0; z <= (base.zf_alt ? 1 : 0); z++) {
+        std::vector<int> zc{0};
+        if (base.zf_alt) zc.push_back(1);
+        if (base.zf_alt2) zc.push_back(4);
+        for (int z : zc) {
             if (base.xstep) cand.push_back(z);
             cand.push_back(z | 2);
         }
@@ -471,25 +495,33 @@ struct SizeClass {
             const hipError_t e = f();
             return e == hipSuccess ? 0 : fail(MWF_E_HIP, hipGetErrorString(e));
         };
-        int best = cand[0];
-        float best_ms = -1.0f;
+        // every candidate runs 2 untimed steps (code load, first-step path),
+        // then NR rounds, interleaved across candidates, of one untimed and NS
+        // timed steps from a fresh zero state; a candidate's time is its best
+        // round, which keeps a single slow round from deciding the choice
+        constexpr int NR = 3, NS = 5;
+        std::vector<float> tbest(cand.size(), -1.0f);
         int rc = 0;
-        for (int ch : cand) {
-            apply(ch);
+        for (size_t i = 0; i < cand.size() && !rc; i++) {
+            apply(cand[i]);
             if ((rc = reset_state(st))) break;
             for (int w = 0; w < 2 && !rc; w++) rc = step(st, run);
-            if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = fail(MWF_E_HIP, "hipEventRecord");
-            for (int w = 0; w < 3 && !rc; w++) rc = step(st, run);
-            if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = fail(MWF_E_HIP, "hipEventRecord");
-            float ms = 0.0f;
-            if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                rc = fail(MWF_E_HIP, "hipEventElapsedTime");
-            if (rc) break;
-            if (best_ms < 0.0f || ms < best_ms) {
-                best_ms = ms;
-                best = ch;
+        }
+        for (int round = 0; round < NR && !rc; round++) {
+            for (size_t i = 0; i < cand.size() && !rc; i++) {
+                apply(cand[i]);
+                if ((rc = reset_state(st))) break;
+                rc = step(st, run);
+                if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = fail(MWF_E_HIP, "hipEventRecord");
+                for (int w = 0; w < NS && !rc; w++) rc = step(st, run);
+                if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = fail(MWF_E_HIP, "hipEventRecord");
+                float ms = 0.0f;
+                if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = fail(MWF_E_HIP, "hipEventElapsedTime");
+                if (!rc && (tbest[i] < 0.0f || ms < tbest[i])) tbest[i] = ms;
             }
         }
+        const int best = rc ? cand[0] : cand[std::min_element(tbest.begin(), tbest.end()) - tbest.begin()];
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc) return rc;
@@ -919,7 +951,7 @@ extern "C" int mwf_blend(mwf_handle* a, mwf_handle* b, double d, mwf_handle* dst
 
 extern "C" int mwf_autotune(int32_t mode)
 {
-    if (mode < 0 || mode > 5) return fail(MWF_E_ARG, "mode must be 0 (rules), 1 (timed) or 2..5 (forced plan)");
+    if (mode < 0 || mode > 9) return fail(MWF_E_ARG, "mode must be 0 (rules), 1 (timed) or 2..9 (forced plan)");
     g_tune.store(mode);
     return 0;
 }
@@ -928,7 +960,8 @@ extern "C" int mwf_plan_info(mwf_handle* h, int32_t* choice)
 {
     if (!h || !choice) return fail(MWF_E_ARG, "null argument");
     const SizeClass& k = *h->cls;
-    *choice = ((k.base.zf_alt && k.plan.zf == k.base.zf_alt) ? 1 : 0) | (k.plan.xstep ? 0 : 2);
+    *choice = ((k.base.zf_alt && k.plan.zf == k.base.zf_alt) ? 1 : 0) | (k.plan.xstep ? 0 : 2) |
+              ((k.base.zf_alt2 && k.plan.zf == k.base.zf_alt2) ? 4 : 0);
     return 0;
 }
 
