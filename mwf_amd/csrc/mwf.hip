@@ -489,8 +489,10 @@ struct SizeClass {
             cand.push_back(z | 2);
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            if (e0) (void)hipEventDestroy(e0);
+            return fail(MWF_E_HIP, "hipEventCreate");
+        }
         auto run = [&](int, auto&& f) -> int {
             const hipError_t e = f();
             return e == hipSuccess ? 0 : fail(MWF_E_HIP, hipGetErrorString(e));
