@@ -421,9 +421,13 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     // waves run their transforms without block barriers and overlap each
     // other's LDS and VALU work; batches alternate between A and B.  Ping-pong
     // form: folds go to A, results come back in B when the pass count is odd.
-    auto seq_fft = [&](CT* buf, auto sign) {
+    // With one x-row per block (XB = 1) the sequences of the valid pairs of a
+    // batch come first, so a batch whose last pairs are padding transforms
+    // only nv pairs (3 nv sequences).
+    auto nseq = [](int nv) { return XB == 1 ? 3 * nv : SEQ; };
+    auto seq_fft = [&](CT* buf, auto sign, int S) {
         const int gl = threadIdx.x % GL;
-        for (int s = threadIdx.x / GL; s < SEQ; s += ZT / GL) {
+        for (int s = threadIdx.x / GL; s < S; s += ZT / GL) {
             CT* sr = buf + s * LP;
             CT v[NN / GL];
 #pragma unroll
@@ -482,12 +486,14 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     __syncthreads();
     static_for<0, NBI>([&](auto bi) {
         constexpr int BN = decltype(bi)::value;
+        constexpr int NV = ZO::NP - BN * ZP < ZP ? ZO::NP - BN * ZP : ZP; // valid pairs
         // fold: Y_b[s] = X[s] w^{sb} + conj(X[NN-s]) w^{-(NN-s)b}, w = exp(2 pi i/Q),
         // two real rows per complex sequence Z = Y(row 1) + i Y(row 2)
 #pragma unroll
         for (int f = 0; f < FT; f++) {
             const int tau = tid + ZT * f;
             const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
+            if (XB == 1 && fp >= NV) continue;
             CT* seq = (GRP && (BN & 1) ? B : A) + ((fx * ZP + fp) * 3) * LP;
 #pragma unroll
             for (int b = 0; b < 3; b++)
@@ -498,10 +504,10 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
         CT* F;
         if constexpr (GRP) {
             F = BN & 1 ? B : A;
-            seq_fft(F, std::integral_constant<int, +1>{});
+            seq_fft(F, std::integral_constant<int, +1>{}, nseq(NV));
             __syncthreads();
         } else {
-            pp_fft<CT, LOGNN, +1, ZT, true>(A, B, SEQ, twNl);
+            pp_fft<CT, LOGNN, +1, ZT, true>(A, B, nseq(NV), twNl);
             F = INB ? B : A;
         }
         // consume: thread (xl, a = r)
@@ -529,6 +535,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     const R scale = (R)g.scale;
     static_for<0, NBO>([&](auto ri) {
         constexpr int RD = decltype(ri)::value;
+        constexpr int NV = NPO - RD * ZP < ZP ? NPO - RD * ZP : ZP; // valid output pairs
         CT* F = GRP && ((NBI + RD) & 1) ? B : A;
         static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
@@ -544,10 +551,10 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
         });
         __syncthreads();
         if constexpr (GRP) {
-            seq_fft(F, std::integral_constant<int, -1>{});
+            seq_fft(F, std::integral_constant<int, -1>{}, nseq(NV));
             __syncthreads();
         } else {
-            pp_fft<CT, LOGNN, -1, ZT, true>(A, B, SEQ, twNl);
+            pp_fft<CT, LOGNN, -1, ZT, true>(A, B, nseq(NV), twNl);
             F = INB ? B : A;
         }
         static_for<0, ZP>([&](auto pi) {
@@ -673,7 +680,11 @@ __global__ __launch_bounds__(Z2T) void k_z2(Batch bt, const cplx_t<R>* __restric
             pf[0] = pf[1] = pf[2] = pf[3] = CT{(R)0, (R)0};
         }
     };
-    auto transform = [&](auto sign) { pp_fft<CT, LOGNN, decltype(sign)::value, Z2T, true>(A, B, SEQ, twNl); };
+    // with one x-row per block only the 3 nv sequences of the valid pairs
+    // of a batch are transformed (the others are padding)
+    auto transform = [&](auto sign, int nv) {
+        pp_fft<CT, LOGNN, decltype(sign)::value, Z2T, true>(A, B, XB == 1 ? 3 * nv : SEQ, twNl);
+    };
 
     R vb[PT][KK];
     R an[PT][KK];
@@ -689,8 +700,9 @@ __global__ __launch_bounds__(Z2T) void k_z2(Batch bt, const cplx_t<R>* __restric
     CT* const F = INB ? B : A; // where a transform's result lies
     static_for<0, NBI>([&](auto bi) {
         constexpr int BN = decltype(bi)::value;
+        constexpr int NV = ZO::NP - BN * ZP < ZP ? ZO::NP - BN * ZP : ZP; // valid pairs
         const int tid = opaque_tid();
-        {
+        if (XB > 1 || (tid / H) % ZP < NV) {
             const int fx = tid / (ZP * H), fp = (tid / H) % ZP, fr = tid % H;
             const int fm = fr == 0 ? H : NN - fr;
             CT* seq = A + ((fx * ZP + fp) * 3) * LP;
@@ -702,7 +714,7 @@ __global__ __launch_bounds__(Z2T) void k_z2(Batch bt, const cplx_t<R>* __restric
         }
         if constexpr (BN + 1 < NBI) prefetch(tid, BN + 1);
         __syncthreads();
-        transform(std::integral_constant<int, +1>{});
+        transform(std::integral_constant<int, +1>{}, NV);
         static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
 #pragma unroll
@@ -727,6 +739,7 @@ __global__ __launch_bounds__(Z2T) void k_z2(Batch bt, const cplx_t<R>* __restric
     const R scale = (R)g.scale;
     static_for<0, NBO>([&](auto ri) {
         constexpr int RD = decltype(ri)::value;
+        constexpr int NV = NPO - RD * ZP < ZP ? NPO - RD * ZP : ZP; // valid output pairs
         const int tid = opaque_tid();
         static_for<0, ZP>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
@@ -741,7 +754,7 @@ __global__ __launch_bounds__(Z2T) void k_z2(Batch bt, const cplx_t<R>* __restric
             }
         });
         __syncthreads();
-        transform(std::integral_constant<int, -1>{});
+        transform(std::integral_constant<int, -1>{}, NV);
 #pragma unroll
         for (int u = 0; u < UT; u++) {
             const int t = tid + Z2T * u;
