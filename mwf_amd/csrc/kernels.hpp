@@ -234,11 +234,11 @@ template <typename R> constexpr int zpairs() { return sizeof(R) == 8 ? 2 : 4; }
 // first, so that every z-derivative can be folded in as it arrives (dz<s>
 // needs only w); base u/v rows are paired with their own z-derivative and
 // loaded once; the x-derivatives (which need only u) come last.  The
-// z-derivatives of w come right after the w rows when DZW (FP64: their
-// second read of the w lines then follows the first by one batch; 2.4 %
-// faster) or after the u/v pairs (FP32, whose 4-pair batches would put both
-// reads in one batch; 2.5 % faster that way).
-template <int K0, bool DZW> struct ZOrder {
+// z-derivatives of w (a second read of the w lines) start at pair D0, any
+// position from NW to NW + NS; the kernels put them at the first pair of the
+// batch after the w rows (zorder_d0), so that the second read follows the
+// first by one batch and still hits L2.
+template <int K0, int D0_> struct ZOrder {
     static_assert(K0 % 2 == 0, "w rows are paired among themselves");
     static constexpr int KK = 3 * K0 - 1;
     static constexpr int NW = K0 / 2;          // pairs of w rows (base, then dz)
@@ -246,14 +246,16 @@ template <int K0, bool DZW> struct ZOrder {
     static constexpr int NX = (KK + 1) / 2;    // pairs of x-derivatives
     static constexpr int NP = 2 * NW + NS + NX;
     static constexpr int W0 = 2 * K0 - 1;      // spec slot of w_0
-    static constexpr int S0 = DZW ? 2 * NW : NW;  // first (base, dz) u/v pair
-    static constexpr int D0 = DZW ? NW : NW + NS; // first dz-w pair
-    __host__ __device__ static constexpr bool shared(int pr) { return pr >= S0 && pr < S0 + NS; }
+    static constexpr int D0 = D0_;                // first dz-w pair
+    static_assert(D0 >= NW && D0 <= NW + NS, "dz-w pairs between the w rows and the dx rows");
+    __host__ __device__ static constexpr bool dzw(int pr) { return pr >= D0 && pr < D0 + NW; }
+    // (base, dz) u/v pairs fill [NW, 2 NW + NS) around the dz-w pairs
+    __host__ __device__ static constexpr bool shared(int pr) { return pr >= NW && pr < 2 * NW + NS && !dzw(pr); }
     __host__ __device__ static constexpr int kind(int q)
     {
         const int pr = q >> 1, h = q & 1;
         if (pr < NW) return 0;
-        if (pr >= D0 && pr < D0 + NW) return 2;
+        if (dzw(pr)) return 2;
         if (shared(pr)) return h == 0 ? 0 : 2;
         if (pr < NP && 2 * (pr - 2 * NW - NS) + h < KK) return 1;
         return 3;
@@ -262,17 +264,25 @@ template <int K0, bool DZW> struct ZOrder {
     {
         const int pr = q >> 1, h = q & 1;
         if (pr < NW) return W0 + 2 * pr + h;
-        if (pr >= D0 && pr < D0 + NW) return W0 + 2 * (pr - D0) + h;
-        if (shared(pr)) return pr - S0;
+        if (dzw(pr)) return W0 + 2 * (pr - D0) + h;
+        if (shared(pr)) return pr < D0 ? pr - NW : pr - 2 * NW;
         return 2 * (pr - 2 * NW - NS) + h;
     }
 };
 
 // Every row appears once; each dz row after all w base rows, each dx row after
 // all u base rows (the order in which the z kernel consumes them).
-template <int K0, bool DZW> constexpr bool zorder_ok()
+// first pair of the batch after the one holding the last w row (batches of ZP pairs)
+template <int K0, int ZP> constexpr int zorder_d0()
 {
-    using ZO = ZOrder<K0, DZW>;
+    constexpr int NW = K0 / 2, NS = 2 * K0 - 1;
+    constexpr int d = (NW + ZP - 1) / ZP * ZP;
+    return d < NW + NS ? d : NW + NS;
+}
+
+template <int K0, int D0> constexpr bool zorder_ok()
+{
+    using ZO = ZOrder<K0, D0>;
     constexpr int KK = ZO::KK;
     int cnt[3][KK] = {};
     int last_w = -1, last_u = -1, first_dz = 1 << 30, first_dx = 1 << 30;
@@ -295,7 +305,8 @@ template <int K0, bool DZW> constexpr bool zorder_ok()
             return false;
     return last_w < first_dz && last_u < first_dx;
 }
-static_assert(zorder_ok<4, true>() && zorder_ok<6, true>() && zorder_ok<4, false>() && zorder_ok<6, false>(),
+static_assert(zorder_ok<4, 2>() && zorder_ok<4, 4>() && zorder_ok<4, 9>() && zorder_ok<6, 3>() &&
+                  zorder_ok<6, 4>() && zorder_ok<6, 14>(),
               "z row order");
 
 // i gamma n v (program/variables.f90:608-609)
@@ -403,7 +414,8 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     constexpr int KK = 3 * K0 - 1;
     constexpr int NN = 1 << LOGNN;
     constexpr int XB = ZT / NN;
-    using ZO = ZOrder<K0, sizeof(R) == 8>;
+    using ZO = ZOrder<K0, zorder_d0<K0, zpairs<R>()>()>;
+    static_assert(zorder_ok<K0, ZO::D0>(), "z row order");
     constexpr int ZP = zpairs<R>();                 // pairs per batch
     constexpr int NBI = (ZO::NP + ZP - 1) / ZP;
     constexpr int NPO = (KK + 1) / 2, NBO = (NPO + ZP - 1) / ZP;
@@ -629,7 +641,8 @@ __global__ __launch_bounds__(Z2T) void k_z2(Batch bt, const cplx_t<R>* __restric
     constexpr int KK = 3 * K0 - 1;
     constexpr int NN = 1 << LOGNN;
     constexpr int XB = 512 / NN;
-    using ZO = ZOrder<K0, sizeof(R) == 8>;
+    using ZO = ZOrder<K0, zorder_d0<K0, Z2P>()>;
+    static_assert(zorder_ok<K0, ZO::D0>(), "z row order");
     constexpr int ZP = Z2P;
     constexpr int NBI = (ZO::NP + ZP - 1) / ZP;
     constexpr int NPO = (KK + 1) / 2, NBO = (NPO + ZP - 1) / ZP;
@@ -833,7 +846,6 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
     U += blockIdx.z * bt.u_stride;
     NL += blockIdx.z * bt.nl_stride;
     using G = Gal<K0>;
-    using ZO = ZOrder<K0, sizeof(R) == 8>;
     constexpr int KK = 3 * K0 - 1;
     constexpr int NN = 1 << LOGNN;
     static_assert(NN <= ZB, "one point per thread");
@@ -841,6 +853,9 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
     constexpr int LP = lpad(NN);
     constexpr int H = NN / 2;
     constexpr int SMAX = zb_seq<R>() / XB;                    // pairs per batch that fit
+    constexpr int NP0 = ZOrder<K0, K0 / 2>::NP;               // pair count (any order)
+    using ZO = ZOrder<K0, zorder_d0<K0, zb_per<NP0, SMAX>()>()>;
+    static_assert(zorder_ok<K0, ZO::D0>(), "z row order");
     constexpr int ZP = zb_per<ZO::NP, SMAX>();                // inverse pairs per batch
     constexpr int NBI = zb_split<ZO::NP, SMAX>();
     constexpr int NPO = (KK + 1) / 2;
