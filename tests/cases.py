@@ -4,7 +4,7 @@ The five workloads are the ones the benchmark is defined on: Model Waleffe
 Flow with the M4/M6 wall-normal bases, periodic box Lx = Lz, seeded
 Gaussian-spectrum initial conditions (mwf_ic_gaussian), parameters.f90
 defaults elsewhere (drag 1e-2, dragall, no hypo-viscosity, theta 0).
-BASELINE "N x N Fourier modes" means i_MM = i_NN = N/2.
+A workload of "N x N Fourier modes" means i_MM = i_NN = N/2.
 """
 import math
 
