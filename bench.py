@@ -249,6 +249,31 @@ def calib_copy(mib):
     print(json.dumps({"calib_copy_mib": mib}), flush=True)
 
 
+PROFILE_CASES = {
+    "m4_1024": (WORKLOAD, IC),
+    "m6_2048": (dict(k0=6, mm=1024, nn=1024, lx=256 * np.pi, lz=256 * np.pi, re=250.0, dt=0.01),
+                dict(seed=3, scl=0.3, k0=8.0)),
+}
+
+
+def profile_case(name, steps, warmup, no_mixed, tune):
+    """A short run of one workload for rocprofv3: FP64, then (unless no_mixed)
+    the mixed form, each with its own handle."""
+    import mwf_amd as mw
+    if tune is not None:
+        mw.autotune(tune)
+    wl, ic = PROFILE_CASES[name]
+    p = mw.make_params(**wl)
+    state = mw.ic_gaussian(p, ic["seed"], ic["scl"], ic["k0"])
+    out = {"profile_case": name}
+    for prec in [mw.MWF_FP64] + ([] if no_mixed else [mw.MWF_MIXED]):
+        r = run_gpu(mw, prec, steps, warmup, state, wl)
+        out["fp64" if prec == mw.MWF_FP64 else "mixed"] = {
+            "steps_per_s": r["steps_per_s"], "plan": r["plan"], "finite": r["finite"],
+            "kernels_ms": {n: round(k["avg_ms"], 5) for n, k in r["kernels"].items()}}
+    print(json.dumps(out), flush=True)
+
+
 def traffic_summary():
     """HBM bytes per k_z launch from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), if any."""
@@ -277,9 +302,16 @@ def main():
                     help="force kernel plan bits (1 k_zb, 2 split x, 4 k_z2) instead of timing the plans")
     ap.add_argument("--cfg5-steps", type=int, default=200)
     ap.add_argument("--calib-copy", type=int, default=0, help="only run a MiB-sized device copy")
+    ap.add_argument("--profile-case", choices=sorted(PROFILE_CASES), default=None,
+                    help="only step one workload (FP64, then mixed) for --steps steps after --warmup, "
+                         "for profiling under rocprofv3")
     args = ap.parse_args()
     if args.calib_copy:
         calib_copy(args.calib_copy)
+        return
+    if args.profile_case:
+        profile_case(args.profile_case, args.steps, args.warmup, args.no_mixed,
+                     None if args.no_tune else (2 + args.plan if args.plan is not None else 1))
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "

@@ -23,7 +23,8 @@
 #include <vector>
 
 #include "../../include/mwf.h"
-#include "kernels.hpp"
+#include "diag.hpp"
+#include "plans.hpp"
 
 static thread_local std::string g_err;
 
@@ -46,230 +47,6 @@ static int fail(int code, const std::string& msg)
     } while (0)
 
 extern "C" const char* mwf_last_error(void) { return g_err.c_str(); }
-
-// ---------------------------------------------------------------------------
-// launchers and dispatch
-// ---------------------------------------------------------------------------
-struct Plan {
-    hipError_t (*xinv)(const Batch&, int, const Geo&, hipStream_t, const double2*, void*, double*, const void*,
-                       const double2*, int) = nullptr;
-    hipError_t (*xfwd)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, const void*,
-                       const double2*, int, int) = nullptr;
-    hipError_t (*zf)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*, const void*) = nullptr;
-    hipError_t (*zphys)(const Batch&, const Geo&, hipStream_t, const void*, double*, const void*, const void*) = nullptr;
-    // fused forward + update + next inverse
-    hipError_t (*xstep)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
-                        const void*, const double2*, int, int) = nullptr;
-    // alternative z kernels the autotuner may pick: k_zb where k_z also runs,
-    // and the two-points-per-thread k_z2
-    hipError_t (*zf_alt)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*,
-                         const void*) = nullptr;
-    hipError_t (*zf_alt2)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*,
-                          const void*) = nullptr;
-};
-
-template <typename F> static void set_lds(F* f, size_t bytes)
-{
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)bytes);
-}
-
-template <typename R, int LOGMM> static size_t x_lds() { return 3 * (size_t)(XTILE >> LOGMM) * lpad(1 << LOGMM) * sizeof(cplx_t<R>); }
-template <typename R, int LOGNN> static size_t zphys_lds()
-{
-    return (size_t)(zphys_threads<LOGNN>() >> LOGNN) * ZPB * 3 * lpad(1 << LOGNN) * sizeof(cplx_t<R>);
-}
-
-template <typename R, int LOGMM>
-static hipError_t launch_xinv(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const double2* c, void* U,
-                              double* ep, const void* twMM, const double2* twP, int addlam)
-{
-    using CT = cplx_t<R>;
-    dim3 grid(g.NN / g.NTx, g.K0, nmem);
-    hipLaunchKernelGGL((k_xinv<R, LOGMM>), grid, dim3(XT), (x_lds<R, LOGMM>()), s, bt, c, (CT*)U, ep, (const CT*)twMM,
-                       twP, addlam);
-    return hipGetLastError();
-}
-
-template <typename R, int LOGMM>
-static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
-                              double2* onl, const void* twMM, const double2* twP, int mode, int nst)
-{
-    using CT = cplx_t<R>;
-    dim3 grid(g.NN / g.NTx, g.K0, nmem);
-    hipLaunchKernelGGL((k_xfwd<R, LOGMM>), grid, dim3(XT), (x_lds<R, LOGMM>()), s, bt, (const CT*)NL, c, onl,
-                       (const CT*)twMM, twP, mode, nst);
-    return hipGetLastError();
-}
-
-// transform buffers, then the pass twiddles of the MM-point transforms and
-// 2 MM + 1 fold twiddles
-template <typename R, int LOGMM> static size_t xstep_lds()
-{
-    constexpr size_t MM = (size_t)1 << LOGMM;
-    return ((xs_of<LOGMM>() / group_lanes<(1 << LOGMM)>()) * MM + ptw_size<(1 << LOGMM)>()) * sizeof(cplx_t<R>) +
-           (2 * MM + 1) * sizeof(double2);
-}
-
-template <typename R, int LOGMM>
-static hipError_t launch_xstep(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
-                               double2* onl, void* U, const void* twMM, const double2* twP, int nst, int inv_only)
-{
-    using CT = cplx_t<R>;
-    dim3 grid(g.NN / g.NTx, g.K0, nmem);
-    if (inv_only)
-        hipLaunchKernelGGL((k_xstep<R, LOGMM, true>), grid, dim3(xs_of<LOGMM>()), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
-                           c, onl, (CT*)U, (const CT*)twMM, twP, nst);
-    else
-        hipLaunchKernelGGL((k_xstep<R, LOGMM, false>), grid, dim3(xs_of<LOGMM>()), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
-                           c, onl, (CT*)U, (const CT*)twMM, twP, nst);
-    return hipGetLastError();
-}
-
-template <typename R, int K0, int LOGNN>
-static hipError_t launch_z(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
-                           const void* twNN, const void* twQ)
-{
-    using CT = cplx_t<R>;
-    constexpr int XB = ZT >> LOGNN;
-    dim3 grid((g.P + XB - 1) / XB, 1, nmem);
-    hipLaunchKernelGGL((k_z<R, K0, LOGNN>), grid, dim3(ZT), (z_lds_bytes<R, K0, LOGNN>()), s, bt, (const CT*)U,
-                       (CT*)NL, (const CT*)twNN, (const CT*)twQ);
-    return hipGetLastError();
-}
-
-template <typename R, int K0, int LOGNN>
-static hipError_t launch_z2(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
-                            const void* twNN, const void* twQ)
-{
-    using CT = cplx_t<R>;
-    constexpr int XB = 512 >> LOGNN;
-    dim3 grid((g.P + XB - 1) / XB, 1, nmem);
-    hipLaunchKernelGGL((k_z2<R, K0, LOGNN>), grid, dim3(Z2T), (z2_lds_bytes<R, LOGNN>()), s, bt,
-                       (const CT*)U, (CT*)NL, (const CT*)twNN, (const CT*)twQ);
-    return hipGetLastError();
-}
-
-template <typename R, int K0, int LOGNN>
-static hipError_t launch_zb(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
-                            const void* twNN, const void* twQ)
-{
-    using CT = cplx_t<R>;
-    constexpr int XB = ZB >> LOGNN;
-    dim3 grid((g.P + XB - 1) / XB, 1, nmem);
-    hipLaunchKernelGGL((k_zb<R, K0, LOGNN>), grid, dim3(ZB), (zb_lds_bytes<R, LOGNN>()), s, bt, (const CT*)U, (CT*)NL,
-                       (const CT*)twNN, (const CT*)twQ);
-    return hipGetLastError();
-}
-
-template <typename R, int K0, int LOGNN>
-static hipError_t launch_zphys(const Batch& bt, const Geo& g, hipStream_t s, const void* U, double* phys,
-                               const void* twNN, const void* twQ)
-{
-    using CT = cplx_t<R>;
-    constexpr int TZ = zphys_threads<LOGNN>();
-    constexpr int XB = TZ >> LOGNN;
-    dim3 grid((g.P + XB - 1) / XB);
-    hipLaunchKernelGGL((k_zphys<R, K0, LOGNN>), grid, dim3(TZ), (zphys_lds<R, LOGNN>()), s, bt, (const CT*)U, phys,
-                       (const CT*)twNN, (const CT*)twQ);
-    return hipGetLastError();
-}
-
-// x kernels depend on MM only, z kernels on (K0, NN) only: dispatch each
-// separately so the instantiation count stays additive.
-template <typename R> static void pick_x(int lm, Plan& p)
-{
-    switch (lm) {
-#define C_(L)                                   \
-    case L:                                     \
-        p.xinv = launch_xinv<R, L>;             \
-        p.xfwd = launch_xfwd<R, L>;             \
-        p.xstep = launch_xstep<R, L>;           \
-        break;
-        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
-#undef C_
-    case 10:
-        p.xinv = launch_xinv<R, 10>;
-        p.xfwd = launch_xfwd<R, 10>;
-        // the fused kernel at MM = 1024 pays off in FP64 (2.16 against 2.66 ms at
-        // M6 2048^2); in FP32 its 16-point lanes cost occupancy and it is slower
-        if constexpr (std::is_same<R, double>::value) p.xstep = launch_xstep<R, 10>;
-        break;
-    default: break;
-    }
-}
-
-template <typename R, int K0> static void pick_z(int ln, Plan& p)
-{
-    switch (ln) {
-#define C_(L)                                   \
-    case L:                                     \
-        p.zf = launch_z<R, K0, L>;              \
-        p.zf_alt2 = launch_z2<R, K0, L>;        \
-        p.zphys = launch_zphys<R, K0, L>;       \
-        break;
-        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
-#undef C_
-    default: break;
-    }
-    switch (ln) { // phase-sequential alternatives for the autotuner
-    case 8: p.zf_alt = launch_zb<R, K0, 8>; break;
-    case 9: p.zf_alt = launch_zb<R, K0, 9>; break;
-    case 10: // one x-row per 1024-thread block: the three fold phases in sequence
-        p.zf = launch_zb<R, K0, 10>;
-        p.zphys = launch_zphys<R, K0, 10>;
-        break;
-    default: break;
-    }
-}
-
-template <typename R> static void prepare_x(int lm)
-{
-    switch (lm) {
-#define C_(L)                                         \
-    case L:                                           \
-        set_lds(k_xinv<R, L>, x_lds<R, L>());        \
-        set_lds(k_xfwd<R, L>, x_lds<R, L>());        \
-        set_lds(k_xstep<R, L, false>, xstep_lds<R, L>()); \
-        set_lds(k_xstep<R, L, true>, xstep_lds<R, L>());  \
-        break;
-        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
-#undef C_
-    case 10:
-        set_lds(k_xinv<R, 10>, x_lds<R, 10>());
-        set_lds(k_xfwd<R, 10>, x_lds<R, 10>());
-        if constexpr (std::is_same<R, double>::value) {
-            set_lds(k_xstep<R, 10, false>, xstep_lds<R, 10>());
-            set_lds(k_xstep<R, 10, true>, xstep_lds<R, 10>());
-        }
-        break;
-    default: break;
-    }
-}
-
-template <typename R, int K0> static void prepare_z(int ln)
-{
-    switch (ln) {
-#define C_(L)                                                     \
-    case L:                                                       \
-        set_lds(k_z<R, K0, L>, z_lds_bytes<R, K0, L>());          \
-        set_lds(k_z2<R, K0, L>, z2_lds_bytes<R, L>());            \
-        set_lds(k_zphys<R, K0, L>, zphys_lds<R, L>());            \
-        break;
-        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
-#undef C_
-    default: break;
-    }
-    switch (ln) {
-    case 8: set_lds(k_zb<R, K0, 8>, zb_lds_bytes<R, 8>()); break;
-    case 9: set_lds(k_zb<R, K0, 9>, zb_lds_bytes<R, 9>()); break;
-    case 10:
-        set_lds(k_zb<R, K0, 10>, zb_lds_bytes<R, 10>());
-        set_lds(k_zphys<R, K0, 10>, zphys_lds<R, 10>());
-        break;
-    default: break;
-    }
-}
 
 static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 static int ilog2(int v)

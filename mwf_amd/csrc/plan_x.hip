@@ -1,0 +1,104 @@
+// plan_x.hip -- launchers of the x kernels for one precision, MWF_R (double
+// or float, set on the compile line).  See plans.hpp.
+#include "kernels.hpp"
+#include "plans.hpp"
+
+#ifndef MWF_R
+#error "compile with -DMWF_R=double or -DMWF_R=float"
+#endif
+
+template <typename R, int LOGMM> static size_t x_lds() { return 3 * (size_t)(XTILE >> LOGMM) * lpad(1 << LOGMM) * sizeof(cplx_t<R>); }
+template <typename R, int LOGMM>
+static hipError_t launch_xinv(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const double2* c, void* U,
+                              double* ep, const void* twMM, const double2* twP, int addlam)
+{
+    using CT = cplx_t<R>;
+    dim3 grid(g.NN / g.NTx, g.K0, nmem);
+    hipLaunchKernelGGL((k_xinv<R, LOGMM>), grid, dim3(XT), (x_lds<R, LOGMM>()), s, bt, c, (CT*)U, ep, (const CT*)twMM,
+                       twP, addlam);
+    return hipGetLastError();
+}
+
+template <typename R, int LOGMM>
+static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
+                              double2* onl, const void* twMM, const double2* twP, int mode, int nst)
+{
+    using CT = cplx_t<R>;
+    dim3 grid(g.NN / g.NTx, g.K0, nmem);
+    hipLaunchKernelGGL((k_xfwd<R, LOGMM>), grid, dim3(XT), (x_lds<R, LOGMM>()), s, bt, (const CT*)NL, c, onl,
+                       (const CT*)twMM, twP, mode, nst);
+    return hipGetLastError();
+}
+
+// transform buffers, then the pass twiddles of the MM-point transforms and
+// 2 MM + 1 fold twiddles
+template <typename R, int LOGMM> static size_t xstep_lds()
+{
+    constexpr size_t MM = (size_t)1 << LOGMM;
+    return ((xs_of<LOGMM>() / group_lanes<(1 << LOGMM)>()) * MM + ptw_size<(1 << LOGMM)>()) * sizeof(cplx_t<R>) +
+           (2 * MM + 1) * sizeof(double2);
+}
+
+template <typename R, int LOGMM>
+static hipError_t launch_xstep(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
+                               double2* onl, void* U, const void* twMM, const double2* twP, int nst, int inv_only)
+{
+    using CT = cplx_t<R>;
+    dim3 grid(g.NN / g.NTx, g.K0, nmem);
+    if (inv_only)
+        hipLaunchKernelGGL((k_xstep<R, LOGMM, true>), grid, dim3(xs_of<LOGMM>()), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
+                           c, onl, (CT*)U, (const CT*)twMM, twP, nst);
+    else
+        hipLaunchKernelGGL((k_xstep<R, LOGMM, false>), grid, dim3(xs_of<LOGMM>()), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
+                           c, onl, (CT*)U, (const CT*)twMM, twP, nst);
+    return hipGetLastError();
+}
+
+template <typename R> void pick_x(int lm, Plan& p)
+{
+    switch (lm) {
+#define C_(L)                                   \
+    case L:                                     \
+        p.xinv = launch_xinv<R, L>;             \
+        p.xfwd = launch_xfwd<R, L>;             \
+        p.xstep = launch_xstep<R, L>;           \
+        break;
+        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
+#undef C_
+    case 10:
+        p.xinv = launch_xinv<R, 10>;
+        p.xfwd = launch_xfwd<R, 10>;
+        // the fused kernel at MM = 1024 pays off in FP64 (2.16 against 2.66 ms at
+        // M6 2048^2); in FP32 its 16-point lanes cost occupancy and it is slower
+        if constexpr (std::is_same<R, double>::value) p.xstep = launch_xstep<R, 10>;
+        break;
+    default: break;
+    }
+}
+
+template <typename R> void prepare_x(int lm)
+{
+    switch (lm) {
+#define C_(L)                                         \
+    case L:                                           \
+        set_lds(k_xinv<R, L>, x_lds<R, L>());        \
+        set_lds(k_xfwd<R, L>, x_lds<R, L>());        \
+        set_lds(k_xstep<R, L, false>, xstep_lds<R, L>()); \
+        set_lds(k_xstep<R, L, true>, xstep_lds<R, L>());  \
+        break;
+        C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
+#undef C_
+    case 10:
+        set_lds(k_xinv<R, 10>, x_lds<R, 10>());
+        set_lds(k_xfwd<R, 10>, x_lds<R, 10>());
+        if constexpr (std::is_same<R, double>::value) {
+            set_lds(k_xstep<R, 10, false>, xstep_lds<R, 10>());
+            set_lds(k_xstep<R, 10, true>, xstep_lds<R, 10>());
+        }
+        break;
+    default: break;
+    }
+}
+
+template void pick_x<MWF_R>(int, Plan&);
+template void prepare_x<MWF_R>(int);
