@@ -69,6 +69,10 @@ int mwf_destroy(mwf_handle *h);
 int mwf_set_state(mwf_handle *h, const double *re, const double *im);
 /* Download vel_c (what io_save_state reads, program/io.f90:257-290). */
 int mwf_get_state(mwf_handle *h, double *re, double *im);
+/* Replace vel_c in place, keeping the AB2 history vel_onl and nst: what
+ * vel_imposesym's symmetry projections do to the state between steps
+ * (program/velocity.f90:68-74, called each step from program/main.f90:22). */
+int mwf_put_state(mwf_handle *h, const double *re, const double *im);
 
 /* nsteps x vel_TS (program/velocity.f90:21-41): nonlinear term, AB2 (forward
  * Euler on the first step) and the diagonal implicit update. */
@@ -85,6 +89,16 @@ int mwf_nonlinear(mwf_handle *h, double *nlre, double *nlim);
 
 /* vel_mpt2phys (program/velocity.f90:84-90): mpt -> phys for any host mpt. */
 int mwf_mpt2phys(mwf_handle *h, const double *re, const double *im, double *phys);
+
+/* tra_spec2phys (program/transform.fftw5.f90:64-87): the 3/2-dealiased
+ * inverse transform of a type(spec) (re, im planes, (KK, M, NN) column-major,
+ * program/variables.f90:17-20) to a type(phys) ((KK, Q, P), :8-10): x c2c of
+ * length P with sign +1, then z c2r of length Q, both unnormalised.  Runs in
+ * the handle's precision. */
+int mwf_spec2phys(mwf_handle *h, const double *re, const double *im, double *phys);
+/* tra_phys2spec (program/transform.fftw5.f90:122-145): scale by 1/(P Q),
+ * z r2c, x c2c with sign -1, truncated to |m| < MM, n < NN. */
+int mwf_phys2spec(mwf_handle *h, const double *phys, double *re, double *im);
 
 /* AB2 history vel_onl (program/velocity.f90:16,24-27,39), for a warm restart
  * that continues AB2 instead of restarting with forward Euler: the state
@@ -113,6 +127,14 @@ int mwf_vel_history(mwf_handle *h, int32_t nh, double y, double *ans);
  * caller keeps the i_MT time history (turb.f90:43, the shift of tdata). */
 int mwf_turb_sample(mwf_handle *h, int32_t dm, int32_t dn, double *out);
 
+/* Turbulent fraction of io_writeVTK_txz (program/io.f90:550-617), as
+ * utils/turbfrac.f90:14-38 drives it, on the device: of the physical field
+ * vel_mpt2phys(vel_c), the number of points ix = 0, xs, ... < P,
+ * iz = 0, zs, ... < Q with epos > ct (program/velocity.f90:115-126), divided
+ * by (N_+1)(M_+1), M_ = P/xs, N_ = Q/zs (the reference's denominator, which
+ * includes the periodic copies its VTK file repeats). */
+int mwf_turb_fraction(mwf_handle *h, int32_t xs, int32_t zs, double ct, double *frac);
+
 /* var_doubleX / var_quadX (program/variables.f90:147-185, utils/doubleX.f90,
  * utils/quadX.f90) on the device: dst (same k0, mm, nn; alpha = src alpha /
  * factor, a box factor times longer in x) receives src's state with mode m
@@ -130,13 +152,17 @@ int mwf_blend(mwf_handle *a, mwf_handle *b, double d, mwf_handle *dst);
  * handle or ensemble class of each (precision, k0, mm, nn) times its
  * candidate kernel plans at creation (a few steps each, on its zero state)
  * and keeps the fastest, and later ones of that size reuse the choice, so
- * results stay reproducible within a process; 2..9 = force plan choice
+ * results stay reproducible within a process; 2..33 = force plan choice
  * mode - 2 where that alternative exists (for tests and plan comparisons). */
 int mwf_autotune(int32_t mode);
 /* The plan a handle runs: bit 0 = phase-sequential z kernel instead of the
  * three-phase one, bit 1 = separate x-inverse and x-forward kernels instead
  * of the fused x kernel, bit 2 = three-phase z kernel with two points per
- * thread (768-thread blocks) instead of three (512). */
+ * thread (768-thread blocks) instead of three (512), bit 3 = the flux
+ * (conservative) form of vel_nonlinear's products: the same projected
+ * (u.grad)u, with d/dx, d/dz taken in spectral space and d/dy moved onto the
+ * wall-normal bases, equal to the advective form up to rounding; bit 4 (with
+ * bit 3) = its z kernel compiled for two waves per SIMD instead of four. */
 int mwf_plan_info(mwf_handle *h, int32_t *choice);
 
 /* Wait for all queued work of the handle. */
@@ -192,12 +218,20 @@ int mwf_ens_energy(mwf_ens *e, double *energies);
 int mwf_ens_sync(mwf_ens *e);
 /* Record every member's energy every `every` steps (0 = off) and the first
  * step at which it is <= threshold (relaminarisation; the reference's
- * io_write_energy kill, program/io.f90:462-465, uses an absolute 1e-10). */
+ * io_write_energy kill, program/io.f90:462-465, uses an absolute 1e-10).
+ * The samples are taken on the device inside the replayed step graphs; the
+ * first one (step 0) is taken at this call. */
 int mwf_ens_track(mwf_ens *e, int32_t every, double threshold);
 /* Member's recorded energies (up to cap), their count, and the
  * relaminarisation step (-1 if none). */
 int mwf_ens_trace(mwf_ens *e, int32_t member, double *energies, int32_t cap, int32_t *count,
                   int64_t *relam_step);
+/* Add every member's turbulent fraction (as mwf_turb_fraction, with strides
+ * xs, zs and threshold ct; utils/turbfrac.f90:14-38) to the tracking samples
+ * of mwf_ens_track, which must be on.  Restarts the tracking. */
+int mwf_ens_track_turb(mwf_ens *e, int32_t xs, int32_t zs, double ct);
+/* Member's recorded turbulent fractions (up to cap) and their count. */
+int mwf_ens_trace_turb(mwf_ens *e, int32_t member, double *fractions, int32_t cap, int32_t *count);
 /* Number of size classes the members were grouped into. */
 int mwf_ens_classes(mwf_ens *e, int32_t *nclasses);
 

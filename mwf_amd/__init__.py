@@ -12,6 +12,9 @@ reference              here
 ``vel_TS``             ``Velocity.vel_TS(nsteps=1)``
 ``vel_nonlinear``      ``Velocity.vel_nonlinear()`` (returns vel_nl)
 ``vel_mpt2phys``       ``Velocity.vel_mpt2phys(re, im)``
+``vel_imposesym``      ``Velocity.put_state`` (replace vel_c, keep AB2)
+``tra_spec2phys``      ``Velocity.tra_spec2phys(re, im)``
+``tra_phys2spec``      ``Velocity.tra_phys2spec(phys)``
 ``vel_energy``         ``Velocity.vel_energy()`` (of the current vel_c)
 ``vel_onl`` (restart)  ``Velocity.get_onl()`` / ``set_onl``
 ``io_save_Uspec``      ``Velocity.spectrum_max()``
@@ -82,6 +85,9 @@ def lib():
         L.mwf_destroy.argtypes = [vp]
         L.mwf_set_state.argtypes = [vp, _dp, _dp]
         L.mwf_get_state.argtypes = [vp, _dp, _dp]
+        L.mwf_put_state.argtypes = [vp, _dp, _dp]
+        L.mwf_spec2phys.argtypes = [vp, _dp, _dp, _dp]
+        L.mwf_phys2spec.argtypes = [vp, _dp, _dp, _dp]
         L.mwf_step.argtypes = [vp, ctypes.c_int32]
         L.mwf_energy.argtypes = [vp, _dp]
         L.mwf_nonlinear.argtypes = [vp, _dp, _dp]
@@ -98,6 +104,7 @@ def lib():
         L.mwf_spectrum_max.argtypes = [vp, _dp, _dp]
         L.mwf_vel_history.argtypes = [vp, ctypes.c_int32, ctypes.c_double, _dp]
         L.mwf_turb_sample.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, _dp]
+        L.mwf_turb_fraction.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp]
         L.mwf_extend_x.argtypes = [vp, ctypes.c_int32, vp]
         L.mwf_blend.argtypes = [vp, vp, ctypes.c_double, vp]
         L.mwf_autotune.argtypes = [ctypes.c_int32]
@@ -119,11 +126,11 @@ def _p(a):
 def autotune(mode=1):
     """Process-wide plan selection for handles created afterwards (mwf_autotune),
     the role of tra_precompute's FFTW_PATIENT planning: 0 = rule-based plans,
-    1 = timed choice per size, 2..9 = force plan bits mode - 2."""
+    1 = timed choice per size, 2..33 = force plan bits mode - 2."""
     _check(lib().mwf_autotune(int(mode)))
 
 
-PLAN_ZB, PLAN_SPLIT_X, PLAN_Z2 = 1, 2, 4
+PLAN_ZB, PLAN_SPLIT_X, PLAN_Z2, PLAN_FLUX, PLAN_FLUX_W2 = 1, 2, 4, 8, 16
 
 
 def dims(p):
@@ -179,6 +186,33 @@ class Velocity:
         return re, im
 
     vel_c = property(get_state)
+
+    def put_state(self, re, im):
+        """Replace vel_c keeping the AB2 history (vel_imposesym's projections,
+        program/velocity.f90:68-74)."""
+        re = np.ascontiguousarray(re, dtype=np.float64).reshape(self.mpt_shape)
+        im = np.ascontiguousarray(im, dtype=np.float64).reshape(self.mpt_shape)
+        _check(lib().mwf_put_state(self._h, _p(re), _p(im)))
+
+    @property
+    def spec_shape(self):
+        return (self.NN, self.M, self.KK)
+
+    def tra_spec2phys(self, re, im):
+        """tra_spec2phys (program/transform.fftw5.f90:64-87): spec (NN, M, KK)
+        planes -> phys (P, Q, KK)."""
+        re = np.ascontiguousarray(re, dtype=np.float64).reshape(self.spec_shape)
+        im = np.ascontiguousarray(im, dtype=np.float64).reshape(self.spec_shape)
+        ph = np.empty((self.P, self.Q, self.KK))
+        _check(lib().mwf_spec2phys(self._h, _p(re), _p(im), _p(ph)))
+        return ph
+
+    def tra_phys2spec(self, ph):
+        """tra_phys2spec (program/transform.fftw5.f90:122-145): phys -> spec planes."""
+        ph = np.ascontiguousarray(ph, dtype=np.float64).reshape((self.P, self.Q, self.KK))
+        re, im = np.empty(self.spec_shape), np.empty(self.spec_shape)
+        _check(lib().mwf_phys2spec(self._h, _p(ph), _p(re), _p(im)))
+        return re, im
 
     def vel_TS(self, nsteps=1):
         _check(lib().mwf_step(self._h, int(nsteps)))
@@ -237,6 +271,14 @@ class Velocity:
         _check(lib().mwf_turb_sample(self._h, int(dm), int(dn), _p(out)))
         return out
 
+    def turb_fraction(self, xs, zs, ct):
+        """Turbulent fraction of io_writeVTK_txz (program/io.f90:550-617,
+        utils/turbfrac.f90): points with epos > ct on the (xs, zs)-strided grid
+        over (Q/zs + 1)(P/xs + 1)."""
+        f = ctypes.c_double()
+        _check(lib().mwf_turb_fraction(self._h, int(xs), int(zs), float(ct), ctypes.byref(f)))
+        return f.value
+
     def plan_info(self):
         """Plan bits: PLAN_ZB (phase-sequential z kernel), PLAN_SPLIT_X (separate x kernels),
         PLAN_Z2 (two-points-per-thread z kernel)."""
@@ -290,6 +332,8 @@ class Ensemble:
             L.mwf_ens_trace.argtypes = [vp, ctypes.c_int32, _dp, ctypes.c_int32,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
             L.mwf_ens_classes.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+            L.mwf_ens_track_turb.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
+            L.mwf_ens_trace_turb.argtypes = [vp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
             L._ens_typed = True
         self.params = list(params_list)
         arr = (Params * len(self.params))(*self.params)
@@ -345,6 +389,17 @@ class Ensemble:
         buf = np.empty(max(cnt.value, 1))
         _check(lib().mwf_ens_trace(self._h, int(i), _p(buf), cnt.value, ctypes.byref(cnt), ctypes.byref(rel)))
         return buf[:cnt.value], rel.value
+
+    def track_turb(self, xs, zs, ct):
+        """Add each member's turbulent fraction to the tracking samples."""
+        _check(lib().mwf_ens_track_turb(self._h, int(xs), int(zs), float(ct)))
+
+    def trace_turb(self, i):
+        cnt = ctypes.c_int32()
+        _check(lib().mwf_ens_trace_turb(self._h, int(i), None, 0, ctypes.byref(cnt)))
+        buf = np.empty(max(cnt.value, 1))
+        _check(lib().mwf_ens_trace_turb(self._h, int(i), _p(buf), cnt.value, ctypes.byref(cnt)))
+        return buf[:cnt.value]
 
     def nclasses(self):
         n = ctypes.c_int32()

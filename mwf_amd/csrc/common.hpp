@@ -71,6 +71,7 @@ struct Geo {
     int NTx;           // z-modes per x-kernel block (power of two)
     int lgNT;          // log2(NTx)
     int XB;            // x-rows per z-kernel block
+    int UL, NLL;       // lines per x-row of U and NL: 2 KK and KK (advective form), KK and 2 KK (flux form)
     double alpha, gamma, re, dt, drag, vdrag, hypo, cth, sth, beta;
     double scale;      // 1/(P*Q), transform.fftw5.f90:242
     int dragall;
@@ -97,24 +98,25 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
     return (b & 7) * per + (b >> 3);
 }
 
-// U (the x-transformed lines, 2*KK per x row) is stored U[x][line][n]: the z
-// kernel reads whole rows.  NL (the z-forward products) is stored in panels
-// NL[tile][slot][x][t], n = tile*NTx + t: the x kernels read whole panels.
+// U (the x-transformed lines, UL per x row) is stored U[x][line][n]: the z
+// kernel reads whole rows.  NL (the z-forward products, NLL lines) is stored
+// in panels NL[tile][line][x][t], n = tile*NTx + t: the x kernels read whole
+// panels.
 // The writers of both store NTx- or NN-long pieces; partial-line writes cost
 // no extra HBM reads, partial-line reads would.
 __device__ __forceinline__ size_t uidx(const Geo& g, int x, int line, int n)
 {
-    return ((size_t)x * 2 * g.KK + line) * g.NN + n;
+    return ((size_t)x * g.UL + line) * g.NN + n;
 }
 // NL panel (tile of n0, slot): P * NTx contiguous values, element x * NTx + t.
 template <typename CT> __device__ __forceinline__ const CT* nl_panel(const Geo& g, const CT* NL, int n0, int slot)
 {
-    return NL + ((size_t)(n0 >> g.lgNT) * g.KK + slot) * g.P * g.NTx;
+    return NL + ((size_t)(n0 >> g.lgNT) * g.NLL + slot) * g.P * g.NTx;
 }
 __device__ __forceinline__ size_t nidx(const Geo& g, int x, int slot, int n)
 {
     const int tile = n >> g.lgNT, t = n & (g.NTx - 1);
-    return (((size_t)tile * g.KK + slot) * g.P + x) * g.NTx + t;
+    return (((size_t)tile * g.NLL + slot) * g.P + x) * g.NTx + t;
 }
 
 // wall-normal derivative table ad_k1, ad_k2 (program/variables.f90:528-531)

@@ -88,28 +88,92 @@ __global__ void k_velhist(const double* __restrict__ phys, int Q, int KK, int K0
     ans[4 * j + 3] = w;
 }
 
-// turb_sample (program/turb.f90:37-47): eposC (program/velocity.f90:128-142)
-// of the physical field at x = m*dm, z = n*dn: the y-means of u^2, v^2, w^2,
-// nluw(u,u)(0) = u0 u0 + (u1 u1)/2 + ..., nlvv(v)(0) = v1**2/2 + ...
-// (modes.M4.f90:38-81).  out is (3, SN, SM) column-major.
-__global__ void k_eposc(const double* __restrict__ phys, int Q, int KK, int K0, int dm, int dn, int SM, int SN,
-                        double* __restrict__ out)
+// eposC (program/velocity.f90:128-142) of one point's KK wall-normal values:
+// the y-means of u^2, v^2, w^2, i.e. nluw(u,u)(0) = u0 u0 + (u1 u1)/2 + ...,
+// nlvv(v)(0) = v1**2/2 + ... (modes.M4.f90:38-81), in the reference's order.
+__device__ __forceinline__ void eposc_of(const double* __restrict__ V, int K0, double& eu, double& ev, double& ew)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= SM * SN) return;
-    const int m = i / SN, n = i - m * SN;
-    const double* V = phys + ((size_t)m * dm * Q + (size_t)n * dn) * KK;
-    double eu = V[0] * V[0], ev = 0.0, ew = V[2 * K0 - 1] * V[2 * K0 - 1];
+    eu = V[0] * V[0];
+    ev = 0.0;
+    ew = V[2 * K0 - 1] * V[2 * K0 - 1];
     for (int k = 1; k < K0; k++) eu = eu + (V[k] * V[k]) / 2;
     for (int k = 1; k < K0; k++) {
         const double t = V[K0 - 1 + k] * V[K0 - 1 + k] / 2;
         ev = k == 1 ? t : ev + t;
     }
     for (int k = 1; k < K0; k++) ew = ew + (V[2 * K0 - 1 + k] * V[2 * K0 - 1 + k]) / 2;
+}
+
+// turb_sample (program/turb.f90:37-47): eposC of the physical field at
+// x = m*dm, z = n*dn.  out is (3, SN, SM) column-major.
+__global__ void k_eposc(const double* __restrict__ phys, int Q, int KK, int K0, int dm, int dn, int SM, int SN,
+                        double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= SM * SN) return;
+    const int m = i / SN, n = i - m * SN;
+    double eu, ev, ew;
+    eposc_of(phys + ((size_t)m * dm * Q + (size_t)n * dn) * KK, K0, eu, ev, ew);
     out[3 * i + 0] = eu;
     out[3 * i + 1] = ev;
     out[3 * i + 2] = ew;
+}
+
+// Turbulent points of io_writeVTK_txz (program/io.f90:550-617, driven by
+// utils/turbfrac.f90:14-38): on the grid iz = 0, zs, ... < Q, ix = 0, xs, ...
+// < P, the points with epos > ct (epos = (eu + ev) + ew, program/velocity.f90
+// :115-126).  blockIdx.y selects the member (phys arrays back to back);
+// count[member] accumulates with integer atomics, so the count does not
+// depend on the order of the updates.
+__global__ void k_turbcount(const double* __restrict__ phys, const Geo* __restrict__ geos, int xs, int zs,
+                            double ct, unsigned long long* __restrict__ count)
+{
+#pragma clang fp contract(off)
+    const Geo& g = geos[blockIdx.y];
+    phys += blockIdx.y * ((size_t)g.KK * g.Q * g.P);
+    const int nx = (g.P + xs - 1) / xs, nz = (g.Q + zs - 1) / zs;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int hit = 0;
+    if (i < nx * nz) {
+        const int ix = (i / nz) * xs, iz = (i % nz) * zs;
+        double eu, ev, ew;
+        eposc_of(phys + ((size_t)ix * g.Q + iz) * g.KK, g.K0, eu, ev, ew);
+        hit = ((eu + ev) + ew) > ct;
+    }
+    // one atomic per wave
+    const unsigned long long wave = __ballot(hit);
+    if ((threadIdx.x & 63) == 0 && wave) atomicAdd(&count[blockIdx.y], (unsigned long long)__popcll(wave));
+}
+
+// One sample of the ensemble's on-device tracking, for the nmem members of
+// a class: energies e[] (vel_energy) and, when count is set, turbulent point
+// counts (k_turbcount, then cleared for the next sample) as fractions
+// count / den.  The sample goes to row *fill of the trace buffers (cap rows
+// of nmem), the first step with energy <= thr to relam[] (step = sample index
+// since tracking began times every); *fill and *total then advance.  The host
+// drains the buffers before they fill.  One block.
+__global__ void k_track_push(int nmem, const double* __restrict__ e, unsigned long long* __restrict__ count,
+                             double den, double thr, long long every, double* __restrict__ etrace,
+                             double* __restrict__ ftrace, int* __restrict__ fill, long long* __restrict__ total,
+                             long long* __restrict__ relam, int cap)
+{
+    const int row = *fill;
+    const long long step = *total * every;
+    for (int i = threadIdx.x; i < nmem; i += blockDim.x) {
+        const double ei = e[i];
+        if (row < cap) {
+            etrace[(size_t)row * nmem + i] = ei;
+            if (count) ftrace[(size_t)row * nmem + i] = (double)count[i] / den;
+        }
+        if (count) count[i] = 0ull;
+        if (relam[i] < 0 && ei <= thr) relam[i] = step;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *fill = row + 1;
+        *total = *total + 1;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -80,7 +80,7 @@ __device__ __forceinline__ void mpt2spec_k(const Geo& g, int k, int mi, int n, d
 // when epart is set, the block's share of the perturbation energy (Parseval
 // of the spec fields without the laminar profile, program/velocity.f90:115-162).
 // ---------------------------------------------------------------------------
-template <typename R, int LOGMM>
+template <typename R, int LOGMM, bool FLUX>
 __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict__ c, cplx_t<R>* __restrict__ U,
                                              double* __restrict__ epart, const cplx_t<R>* __restrict__ twMM,
                                              const double2* __restrict__ twP, int addlam)
@@ -169,8 +169,9 @@ __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict
     }
     if (!U) return;
 
+    // advective form: u, v, w and their x-derivatives; flux form: u, v, w only
 #pragma unroll
-    for (int li = 0; li < 2 * 3; li++) {
+    for (int li = 0; li < (FLUX ? 3 : 6); li++) {
         const int comp = li % 3, deriv = li / 3;
         if (nlines3 == 2 && comp == 1) continue;
         const int line = spec_slot[comp] + deriv * g.KK;
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict
         ip_fft<CT, LOGMM, +1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);
         // write U[x][line][n0 + t]
         CT* Ul = U + (size_t)line * g.NN + n0;
-        const int ustride = 2 * g.KK * g.NN;
+        const int ustride = g.UL * g.NN;
 #pragma unroll
         for (int q = 0; q < 3 * IPT; q++) {
             const int it = tid + q * XT;
@@ -990,14 +991,17 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
     }
 }
 
-// z inverse only, to the reference phys layout (vel_mpt2phys).
+// z inverse only, to the reference phys layout (vel_mpt2phys); blockIdx.y
+// selects the member, whose phys array follows the previous one's.
 template <int LOGNN> constexpr int zphys_threads() { return (1 << LOGNN) > ZT ? (1 << LOGNN) : ZT; }
 template <typename R, int K0, int LOGNN>
 __global__ __launch_bounds__(zphys_threads<LOGNN>()) void k_zphys(Batch bt, const cplx_t<R>* __restrict__ U, double* __restrict__ phys,
                                               const cplx_t<R>* __restrict__ twNN, const cplx_t<R>* __restrict__ twQ)
 {
     using CT = cplx_t<R>;
-    const Geo g = bt.geo[0];
+    const Geo g = bt.geo[blockIdx.y];
+    U += blockIdx.y * bt.u_stride;
+    phys += blockIdx.y * ((size_t)g.KK * g.Q * g.P);
     constexpr int KK = 3 * K0 - 1;
     constexpr int NN = 1 << LOGNN;
     constexpr int TZ = zphys_threads<LOGNN>();
@@ -1124,7 +1128,7 @@ __device__ __forceinline__ double2 ab2_update(double2 cv, double2 nl, double2 ov
 // mode 0: AB2/implicit update of c and onl (vel_TS); nst = first step.
 // mode 1: write the nonlinear term into onl only (vel_nonlinear).
 // ---------------------------------------------------------------------------
-template <typename R, int LOGMM>
+template <typename R, int LOGMM, bool FLUX>
 __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
                                              double2* __restrict__ onl, const cplx_t<R>* __restrict__ twMM,
                                              const double2* __restrict__ twP, int mode, int nst)
@@ -1147,13 +1151,16 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
     const int nitems = NT * MM;
     const double ak1 = ak1_of(k, g.beta);
 
-    // psi[] holds u_k until w_k arrives, phi[] holds v_k (am2+an2) until then
+    // psi[] holds u_k until w_k arrives, phi[] holds v_k (am2+an2) until then;
+    // xd[] holds i alpha m B of the flux form until its A line arrives
     double2 psi[IPT][2], phi[IPT][2];
+    double2 xd[FLUX ? IPT : 1][2];
 
-    for (int comp = 0; comp < 3; comp++) {
+    for (int li = 0; li < (FLUX ? 6 : 3); li++) {
+        const int comp = FLUX ? li >> 1 : li, part = FLUX ? li & 1 : 0;
         if (k == 0 && comp == 1) continue;
         const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
-        const CT* pan = nl_panel(g, NL, n0, slot);
+        const CT* pan = nl_panel(g, NL, n0, slot + part * g.KK);
         // load NL rows, decimated by b = x mod 3
 #pragma unroll
         for (int q = 0; q < 3 * IPT; q++) {
@@ -1191,12 +1198,20 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
             }
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const double2 X = h == 0 ? Xp : Xn;
+                const double2 X0 = h == 0 ? Xp : Xn;
                 if (h == 1 && r == 0) continue;
                 const int mi = h == 0 ? r : MM - 1 + r;
                 double am1, am2, an1, an2;
                 xwave(g, mi, am1, am2);
                 zwave(g, n, an1, an2);
+                double2 X = X0;
+                if constexpr (FLUX) { // N = i alpha m B + A
+                    if (part == 0) {
+                        xd[q][h] = double2{-(X.y * am1), X.x * am1};
+                        continue;
+                    }
+                    X = double2{xd[q][h].x + X.x, xd[q][h].y + X.y};
+                }
                 // var_spec2mpt (program/variables.f90:440-480)
                 if (comp == 0) {
                     psi[q][h] = X;
@@ -1265,7 +1280,7 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
 // ---------------------------------------------------------------------------
 template <int LOGMM> constexpr int xs_of() { return 3 * (XTILE >> LOGMM) * group_lanes<(1 << LOGMM)>(); }
 
-template <typename R, int LOGMM, bool INV_ONLY>
+template <typename R, int LOGMM, bool INV_ONLY, bool FLUX>
 __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
                                               double2* __restrict__ onl, cplx_t<R>* __restrict__ U,
                                               const cplx_t<R>* __restrict__ twMM, const double2* __restrict__ twP,
@@ -1309,8 +1324,10 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     const double ak2 = ak2_of(k, g.beta);
 
     // psi[] holds u_k until w_k arrives, phi[] holds v_k (am2+an2); after the
-    // update both hold the new state
+    // update both hold the new state.  xd[] holds i alpha m B of the flux
+    // form until its A line arrives.
     double2 psi[IPT][2], phi[IPT][2];
+    double2 xd[FLUX ? IPT : 1][2];
 
     ptw_fill<CT, LOGMM>(twMl, twMM, tid, XS);
     for (int i = tid; i <= 2 * MM; i += XS) twPl[i] = twP[i];
@@ -1318,13 +1335,14 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     if constexpr (!INV_ONLY) {
         // ---------------- forward x transform + spec2mpt ----------------
 #pragma unroll 1
-        for (int comp = 0; comp < 3; comp++) {
+        for (int li = 0; li < (FLUX ? 6 : 3); li++) {
+            const int comp = FLUX ? li >> 1 : li, part = FLUX ? li & 1 : 0;
             if (k == 0 && comp == 1) continue;
             const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
             // opaque copy of the thread index: keeps item arithmetic inside the loop
             int tidv = tid;
             asm volatile("" : "+v"(tidv));
-            const CT* pan = nl_panel(g, NL, n0, slot);
+            const CT* pan = nl_panel(g, NL, n0, slot + part * g.KK);
 #pragma unroll
             for (int q = 0; q < LPT; q++) {
                 const int i = tidv + XS * q;
@@ -1366,11 +1384,18 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     if (h == 1 && r == 0) continue;
-                    const double2 X = h == 0 ? Xp : Xn;
+                    double2 X = h == 0 ? Xp : Xn;
                     const int mi = h == 0 ? r : MM - 1 + r;
                     double am1, am2, an1, an2;
                     xwave(g, mi, am1, am2);
                     zwave(g, n, an1, an2);
+                    if constexpr (FLUX) { // N = i alpha m B + A
+                        if (part == 0) {
+                            xd[q][h] = double2{-(X.y * am1), X.x * am1};
+                            continue;
+                        }
+                        X = double2{xd[q][h].x + X.x, xd[q][h].y + X.y};
+                    }
                     if (comp == 0) {
                         psi[q][h] = X;
                     } else if (comp == 1) {
@@ -1427,7 +1452,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     const bool wide = g.NTx * (int)sizeof(CT) >= 64;
     // ---------------- inverse x transform of the new state ----------------
 #pragma unroll 1
-    for (int li = 0; li < 6; li++) {
+    for (int li = 0; li < (FLUX ? 3 : 6); li++) {
         const int comp = li % 3, deriv = li / 3;
         if (k == 0 && comp == 1) continue;
         // spec slot of u_k, v_k, w_k: k, K0-1+k, 2K0-1+k
@@ -1491,7 +1516,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
         }
         __syncthreads();
         CT* Ul = U + (size_t)line * g.NN + n0;
-        const int ustride = 2 * g.KK * g.NN;
+        const int ustride = g.UL * g.NN;
 #pragma unroll
         for (int q = 0; q < LPT; q++) {
             const int i = tidv + XS * q;

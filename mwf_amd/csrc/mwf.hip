@@ -105,6 +105,8 @@ static Geo geo_of(const mwf_params* p)
     g.NTx = std::min(p->nn, XTILE / p->mm);
     g.lgNT = ilog2(g.NTx);
     g.XB = ZT / p->nn;
+    g.UL = 2 * g.KK;
+    g.NLL = g.KK;
     g.alpha = p->alpha;
     g.gamma = p->gamma;
     g.re = p->re;
@@ -153,7 +155,10 @@ template <typename T> static int dev_upload(T** d, const T* h, size_t n)
 // mode - 2 (where the alternative exists), for tests and plan comparisons.
 // Choice bits: 1 = phase-sequential z kernel (k_zb) instead of k_z, 2 =
 // x-inverse + x-forward instead of the fused x kernel, 4 = two-points-per-
-// thread z kernel (k_z2; takes precedence over bit 1).
+// thread z kernel (k_z2; takes precedence over bit 1), 8 = flux form of the
+// nonlinear term (k_zc and the flux x kernels; takes precedence over bits 1
+// and 4, where it exists), 16 = with bit 8, the flux z kernel compiled for
+// 2 waves per SIMD (no spills) instead of 4 (two blocks per CU).
 // ---------------------------------------------------------------------------
 static std::atomic<int> g_tune{0};
 static std::mutex g_tune_mu;
@@ -197,10 +202,12 @@ struct SizeClass {
         g0 = geo_of(&ps[0]);
         for (auto& p : ps) geos_h.push_back(geo_of(&p));
         plan = base = plan_for(prec, g0.K0, g0.MM, g0.NN);
-        if (!plan.xinv || !plan.xfwd || !plan.zf || !plan.zphys) return fail(MWF_E_ARG, "no kernel for this size");
+        if (!plan.xinv || !plan.xfwd || !plan.zf || !plan.zphys || !plan.sxinv || !plan.sxfwd || !plan.zfwdp)
+            return fail(MWF_E_ARG, "no kernel for this size");
         const size_t rsz = prec == MWF_MIXED ? sizeof(float2) : sizeof(double2);
         mpt_elems = (size_t)g0.NN * g0.K * g0.ldM;
-        const size_t u_el = (size_t)g0.P * 2 * g0.KK * g0.NN, nl_el = (size_t)g0.P * g0.KK * g0.NN;
+        // room for either form: U and NL each hold up to 2 KK lines per x-row
+        const size_t u_el = (size_t)g0.P * 2 * g0.KK * g0.NN, nl_el = (size_t)g0.P * 2 * g0.KK * g0.NN;
         nepart = (g0.NN / g0.NTx) * g0.K0;
         RCCHK(dev_upload(&geos, geos_h.data(), geos_h.size()));
         HIPCHK(hipMalloc((void**)&c, sizeof(double2) * mpt_elems * nmem));
@@ -225,18 +232,43 @@ struct SizeClass {
         HIPCHK(hipStreamSynchronize(st));
         const int mode = g_tune.load();
         if (mode == 1) RCCHK(tune(st));
-        else if (mode >= 2) apply(mode - 2);
+        else if (mode >= 2) RCCHK(apply(mode - 2));
         return 0;
     }
 
-    void apply(int ch)
+    // the plan of choice bits ch; the flux form also changes how many lines U
+    // and NL hold per x-row, in every member's geometry
+    int apply(int ch)
     {
         choice = ch;
         plan = base;
-        if ((ch & 4) && base.zf_alt2) plan.zf = base.zf_alt2;
-        else if ((ch & 1) && base.zf_alt) plan.zf = base.zf_alt;
+        const bool flux = (ch & 8) && base.zc;
+        if (flux) {
+            plan.zf = (ch & 16) && base.zc_alt ? base.zc_alt : base.zc;
+            plan.xinv = base.xinv_f;
+            plan.xfwd = base.xfwd_f;
+            plan.xstep = base.xstep_f;
+        } else if ((ch & 4) && base.zf_alt2) {
+            plan.zf = base.zf_alt2;
+        } else if ((ch & 1) && base.zf_alt) {
+            plan.zf = base.zf_alt;
+        }
         if (ch & 2) plan.xstep = nullptr;
+        const int UL = flux ? g0.KK : 2 * g0.KK, NLL = flux ? 2 * g0.KK : g0.KK;
+        if (g0.UL != UL) {
+            g0.UL = UL;
+            g0.NLL = NLL;
+            for (auto& g : geos_h) {
+                g.UL = UL;
+                g.NLL = NLL;
+            }
+            HIPCHK(hipMemcpy(geos, geos_h.data(), sizeof(Geo) * geos_h.size(), hipMemcpyHostToDevice));
+            u_valid = false;
+        }
+        return 0;
     }
+
+    bool flux() const { return g0.UL == g0.KK; }
 
     int reset_state(hipStream_t st)
     {
@@ -253,16 +285,15 @@ struct SizeClass {
         const std::array<int, 4> key{prec, g0.K0, g0.MM, g0.NN};
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = g_tune_cache.find(key);
-        if (it != g_tune_cache.end()) {
-            apply(it->second);
-            return 0;
-        }
+        if (it != g_tune_cache.end()) return apply(it->second);
         std::vector<int> cand;
         std::vector<int> zc{0};
         if (base.zf_alt) zc.push_back(1);
         if (base.zf_alt2) zc.push_back(4);
+        if (base.zc) zc.push_back(8);
+        if (base.zc_alt) zc.push_back(8 | 16);
         for (int z : zc) {
-            if (base.xstep) cand.push_back(z);
+            if ((z & 8) ? (bool)base.xstep_f : (bool)base.xstep) cand.push_back(z);
             cand.push_back(z | 2);
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -282,13 +313,13 @@ struct SizeClass {
         std::vector<float> tbest(cand.size(), -1.0f);
         int rc = 0;
         for (size_t i = 0; i < cand.size() && !rc; i++) {
-            apply(cand[i]);
+            if ((rc = apply(cand[i]))) break;
             if ((rc = reset_state(st))) break;
             for (int w = 0; w < 2 && !rc; w++) rc = step(st, run);
         }
         for (int round = 0; round < NR && !rc; round++) {
             for (size_t i = 0; i < cand.size() && !rc; i++) {
-                apply(cand[i]);
+                if ((rc = apply(cand[i]))) break;
                 if ((rc = reset_state(st))) break;
                 rc = step(st, run);
                 if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = fail(MWF_E_HIP, "hipEventRecord");
@@ -304,7 +335,7 @@ struct SizeClass {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc) return rc;
-        apply(best);
+        RCCHK(apply(best));
         RCCHK(reset_state(st));
         g_tune_cache[key] = best;
         return 0;
@@ -343,28 +374,39 @@ struct SizeClass {
     }
 };
 
-// reference mpt (re, im planes, (K, M, NN) column-major) <-> device rows
-static void to_dev_layout(const Geo& g, size_t elems, const double* re, const double* im, double2* d)
+// reference mpt / spec (re, im planes, (slots, M, NN) column-major) <-> device
+// rows [NN][slots][ldM]
+static void to_dev_rows(const Geo& g, int slots, size_t elems, const double* re, const double* im, double2* d)
 {
     std::memset(d, 0, sizeof(double2) * elems);
     for (int n = 0; n < g.NN; n++)
         for (int m = 0; m < g.M; m++)
-            for (int k = 0; k < g.K; k++) {
-                const size_t i = (size_t)k + (size_t)g.K * ((size_t)m + (size_t)g.M * n);
-                d[((size_t)n * g.K + k) * g.ldM + m] = double2{re[i], im[i]};
+            for (int k = 0; k < slots; k++) {
+                const size_t i = (size_t)k + (size_t)slots * ((size_t)m + (size_t)g.M * n);
+                d[((size_t)n * slots + k) * g.ldM + m] = double2{re[i], im[i]};
             }
+}
+
+static void from_dev_rows(const Geo& g, int slots, const double2* d, double* re, double* im)
+{
+    for (int n = 0; n < g.NN; n++)
+        for (int m = 0; m < g.M; m++)
+            for (int k = 0; k < slots; k++) {
+                const size_t i = (size_t)k + (size_t)slots * ((size_t)m + (size_t)g.M * n);
+                const double2 v = d[((size_t)n * slots + k) * g.ldM + m];
+                re[i] = v.x;
+                im[i] = v.y;
+            }
+}
+
+static void to_dev_layout(const Geo& g, size_t elems, const double* re, const double* im, double2* d)
+{
+    to_dev_rows(g, g.K, elems, re, im, d);
 }
 
 static void from_dev_layout(const Geo& g, const double2* d, double* re, double* im)
 {
-    for (int n = 0; n < g.NN; n++)
-        for (int m = 0; m < g.M; m++)
-            for (int k = 0; k < g.K; k++) {
-                const size_t i = (size_t)k + (size_t)g.K * ((size_t)m + (size_t)g.M * n);
-                const double2 v = d[((size_t)n * g.K + k) * g.ldM + m];
-                re[i] = v.x;
-                im[i] = v.y;
-            }
+    from_dev_rows(g, g.K, d, re, im);
 }
 
 // ---------------------------------------------------------------------------
@@ -445,6 +487,8 @@ struct mwf_handle {
     void* diag = nullptr;      // device scratch of the diagnostics
     size_t diag_bytes = 0;
     double2* hstage = nullptr; // pinned host staging, one device mpt
+    double2* tspec = nullptr;  // device spec of the transform entry points
+    double2* hspec = nullptr;  // pinned host staging of one spec
     Profiler prof;
 };
 
@@ -457,6 +501,8 @@ static void destroy(mwf_handle* h)
     if (h->phys) (void)hipFree(h->phys);
     if (h->diag) (void)hipFree(h->diag);
     if (h->hstage) (void)hipHostFree(h->hstage);
+    if (h->tspec) (void)hipFree(h->tspec);
+    if (h->hspec) (void)hipHostFree(h->hspec);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
 }
@@ -523,6 +569,14 @@ extern "C" int mwf_get_state(mwf_handle* h, double* re, double* im)
     return download_mpt(h, h->cls->c, re, im);
 }
 
+extern "C" int mwf_put_state(mwf_handle* h, const double* re, const double* im)
+{
+    if (!h || !re || !im) return fail(MWF_E_ARG, "null argument");
+    RCCHK(upload_mpt(h, re, im, h->cls->c));
+    h->cls->u_valid = false; // the AB2 history and nst are kept
+    return 0;
+}
+
 extern "C" int mwf_step(mwf_handle* h, int32_t nsteps)
 {
     if (!h) return fail(MWF_E_ARG, "null handle");
@@ -538,6 +592,9 @@ extern "C" int mwf_nonlinear(mwf_handle* h, double* nlre, double* nlim)
     SizeClass& k = *h->cls;
     HIPCHK(hipMemsetAsync(k.cs, 0, sizeof(double2) * k.mpt_elems, h->st));
     auto launch = [&](hipError_t e) { return e == hipSuccess ? 0 : fail(MWF_E_HIP, hipGetErrorString(e)); };
+    // U is rebuilt here by k_xinv; the next step rebuilds it with its own
+    // inverse, so that stepping does not depend on this call
+    k.u_valid = false;
     RCCHK(launch(k.plan.xinv(k.bt, 1, k.g0, h->st, k.c, k.U, nullptr, k.twMM(), k.twPd, 1)));
     RCCHK(launch(k.plan.zf(k.bt, 1, k.g0, h->st, k.U, k.NL, k.twNN(), k.twQ())));
     // mode 1 writes the nonlinear term into its history argument (the scratch mpt)
@@ -583,6 +640,14 @@ static int ensure_diag(mwf_handle* h, size_t bytes)
     return 0;
 }
 
+// Denominator of the turbulent fraction of io_writeVTK_txz (program/io.f90
+// :564-565,615): (N_+1)(M_+1), M_ = 3M/xs, N_ = 3N/zs, which counts the
+// periodic copies of the first row and column that the VTK file repeats.
+static double turb_den(const Geo& g, int xs, int zs)
+{
+    return (double)(g.Q / zs + 1) * (double)(g.P / xs + 1);
+}
+
 // vel_mpt2phys of a device mpt into h->phys (program/velocity.f90:84-90):
 // var_mpt2spec without the laminar profile, then tra_spec2phys.  U is
 // overwritten, so the next step re-derives it from c.
@@ -594,7 +659,7 @@ static int phys_of(mwf_handle* h, const double2* src)
     k.u_valid = false;
     hipError_t e = k.plan.xinv(k.bt, 1, g, h->st, src, k.U, nullptr, k.twMM(), k.twPd, 0);
     if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
-    e = k.plan.zphys(k.bt, g, h->st, k.U, h->phys, k.twNN(), k.twQ());
+    e = k.plan.zphys(k.bt, 1, g, h->st, k.U, h->phys, k.twNN(), k.twQ());
     if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
     return 0;
 }
@@ -608,6 +673,53 @@ extern "C" int mwf_mpt2phys(mwf_handle* h, const double* re, const double* im, d
     RCCHK(phys_of(h, k.cs));
     HIPCHK(hipMemcpyAsync(phys, h->phys, sizeof(double) * g.KK * g.Q * g.P, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+static int ensure_tspec(mwf_handle* h)
+{
+    const Geo& g = h->cls->g0;
+    const size_t n = (size_t)g.NN * g.KK * g.ldM;
+    if (!h->tspec) HIPCHK(hipMalloc((void**)&h->tspec, sizeof(double2) * n));
+    if (!h->hspec && hipHostMalloc((void**)&h->hspec, sizeof(double2) * n, hipHostMallocDefault) != hipSuccess)
+        return fail(MWF_E_NOMEM, "hipHostMalloc failed");
+    return 0;
+}
+
+extern "C" int mwf_spec2phys(mwf_handle* h, const double* re, const double* im, double* phys)
+{
+    if (!h || !re || !im || !phys) return fail(MWF_E_ARG, "null argument");
+    SizeClass& k = *h->cls;
+    const Geo& g = k.g0;
+    RCCHK(ensure_tspec(h));
+    RCCHK(ensure_phys(h));
+    const size_t n = (size_t)g.NN * g.KK * g.ldM;
+    to_dev_rows(g, g.KK, n, re, im, h->hspec);
+    HIPCHK(hipMemcpyAsync(h->tspec, h->hspec, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+    k.u_valid = false; // U is overwritten
+    hipError_t e = k.plan.sxinv(g, h->st, h->tspec, k.U, k.twMM(), k.twPd);
+    if (e == hipSuccess) e = k.plan.zphys(k.bt, 1, g, h->st, k.U, h->phys, k.twNN(), k.twQ());
+    if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
+    HIPCHK(hipMemcpyAsync(phys, h->phys, sizeof(double) * g.KK * g.Q * g.P, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return 0;
+}
+
+extern "C" int mwf_phys2spec(mwf_handle* h, const double* phys, double* re, double* im)
+{
+    if (!h || !phys || !re || !im) return fail(MWF_E_ARG, "null argument");
+    SizeClass& k = *h->cls;
+    const Geo& g = k.g0;
+    RCCHK(ensure_tspec(h));
+    RCCHK(ensure_phys(h));
+    const size_t n = (size_t)g.NN * g.KK * g.ldM;
+    HIPCHK(hipMemcpyAsync(h->phys, phys, sizeof(double) * g.KK * g.Q * g.P, hipMemcpyHostToDevice, h->st));
+    hipError_t e = k.plan.zfwdp(g, h->st, h->phys, k.NL, k.twNN(), k.twQ());
+    if (e == hipSuccess) e = k.plan.sxfwd(g, h->st, k.NL, h->tspec, k.twMM(), k.twPd);
+    if (e != hipSuccess) return fail(MWF_E_HIP, hipGetErrorString(e));
+    HIPCHK(hipMemcpyAsync(h->hspec, h->tspec, sizeof(double2) * n, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    from_dev_rows(g, g.KK, h->hspec, re, im);
     return 0;
 }
 
@@ -730,7 +842,7 @@ extern "C" int mwf_blend(mwf_handle* a, mwf_handle* b, double d, mwf_handle* dst
 
 extern "C" int mwf_autotune(int32_t mode)
 {
-    if (mode < 0 || mode > 9) return fail(MWF_E_ARG, "mode must be 0 (rules), 1 (timed) or 2..9 (forced plan)");
+    if (mode < 0 || mode > 33) return fail(MWF_E_ARG, "mode must be 0 (rules), 1 (timed) or 2..33 (forced plan)");
     g_tune.store(mode);
     return 0;
 }
@@ -740,7 +852,8 @@ extern "C" int mwf_plan_info(mwf_handle* h, int32_t* choice)
     if (!h || !choice) return fail(MWF_E_ARG, "null argument");
     const SizeClass& k = *h->cls;
     *choice = ((k.base.zf_alt && k.plan.zf == k.base.zf_alt) ? 1 : 0) | (k.plan.xstep ? 0 : 2) |
-              ((k.base.zf_alt2 && k.plan.zf == k.base.zf_alt2) ? 4 : 0);
+              ((k.base.zf_alt2 && k.plan.zf == k.base.zf_alt2) ? 4 : 0) | (k.flux() ? 8 : 0) |
+              ((k.base.zc_alt && k.plan.zf == k.base.zc_alt) ? 16 : 0);
     return 0;
 }
 
@@ -798,39 +911,109 @@ extern "C" int mwf_kernel_bytes(mwf_handle* h, int32_t kernel, double* bytes)
     return 0;
 }
 
+extern "C" int mwf_turb_fraction(mwf_handle* h, int32_t xs, int32_t zs, double ct, double* frac)
+{
+    if (!h || !frac) return fail(MWF_E_ARG, "null argument");
+    SizeClass& k = *h->cls;
+    const Geo& g = k.g0;
+    if (xs < 1 || zs < 1 || xs > g.P || zs > g.Q) return fail(MWF_E_ARG, "strides must be in [1, P] x [1, Q]");
+    RCCHK(phys_of(h, k.c));
+    RCCHK(ensure_diag(h, sizeof(unsigned long long)));
+    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(h->diag);
+    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), h->st));
+    const int npts = ((g.P + xs - 1) / xs) * ((g.Q + zs - 1) / zs);
+    hipLaunchKernelGGL(k_turbcount, dim3((npts + 255) / 256, 1), dim3(256), 0, h->st, h->phys, k.geos, (int)xs,
+                       (int)zs, ct, cnt);
+    HIPCHK(hipGetLastError());
+    unsigned long long c = 0;
+    HIPCHK(hipMemcpyAsync(&c, cnt, sizeof(c), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    *frac = (double)c / turb_den(g, xs, zs);
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // ensemble
+//
+// Each size class steps on its own stream (default priority).  A chunk of
+// steps forks from the ensemble stream to every class stream and joins back
+// at its end, so under stream capture the classes become parallel branches
+// of one graph.  Tracking samples (energies, turbulent fractions,
+// relaminarisation steps) are taken on the device inside the chunk and kept
+// in device trace buffers, which the host drains only when they fill or when
+// it reads them: a tracked run has no host round trip per sample.
 // ---------------------------------------------------------------------------
+struct ClassTrack {
+    double* etrace = nullptr;          // [cap][nmem] energies
+    double* ftrace = nullptr;          // [cap][nmem] turbulent fractions
+    int* fill = nullptr;               // rows written since the last drain
+    long long* total = nullptr;        // samples since tracking began
+    long long* relam = nullptr;        // [nmem] first step with E <= thr, or -1
+    unsigned long long* count = nullptr; // [nmem] turbulent points of the sample
+    void* Ud = nullptr;                // U of the turbulence sample (the step's U stays valid)
+    double* physd = nullptr;           // [nmem] phys of the turbulence sample
+
+    void release()
+    {
+        void* d[] = {etrace, ftrace, fill, total, relam, count, Ud, physd};
+        for (void* p : d)
+            if (p) (void)hipFree(p);
+        *this = ClassTrack{};
+    }
+};
+
 struct mwf_ens {
     hipStream_t st = nullptr;
+    hipEvent_t fork = nullptr;
     std::vector<std::unique_ptr<SizeClass>> classes;
+    std::vector<hipStream_t> cst;           // per class
+    std::vector<hipEvent_t> cev;            // per class, joins
+    std::vector<ClassTrack> trk;            // per class
     std::vector<std::pair<int, int>> where; // member -> (class, index in class)
     std::vector<mwf_params> params;
     double2* hstage = nullptr;
     size_t hstage_elems = 0;
     double* ebuf = nullptr; // host, one energy per member
-    // step graphs by length (steps after the first), built on demand
-    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    // step graphs by (steps, sample at the end), built on demand
+    std::map<std::pair<int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     int nst_all = 1;
     std::vector<std::vector<double>> eclass; // host copy of each class's energies
-    // on-device energy tracking
+    // on-device tracking
+    static constexpr int CAP = 256; // samples held on the device between drains
     int track_every = 0;
     double track_thr = 0.0;
+    int turb_on = 0, turb_xs = 1, turb_zs = 1;
+    double turb_ct = 0.0;
+    int pending = 0;                // samples in the device buffers (all classes alike)
     int64_t step_count = 0;
-    std::vector<std::vector<double>> trace; // [member][sample]
-    std::vector<int64_t> relam;             // first step with E <= threshold, or -1
+    std::vector<std::vector<double>> trace;  // [member][sample] energies
+    std::vector<std::vector<double>> ftrace; // [member][sample] turbulent fractions
+    std::vector<int64_t> relam;              // first step with E <= threshold, or -1
 };
 
-static void ens_destroy(mwf_ens* e)
+static void ens_drop_graphs(mwf_ens* e)
 {
-    if (!e) return;
-    if (e->st) (void)hipStreamSynchronize(e->st);
     for (auto& kv : e->graphs) {
         (void)hipGraphExecDestroy(kv.second.second);
         (void)hipGraphDestroy(kv.second.first);
     }
     e->graphs.clear();
+}
+
+static void ens_destroy(mwf_ens* e)
+{
+    if (!e) return;
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    for (auto s : e->cst)
+        if (s) (void)hipStreamSynchronize(s);
+    ens_drop_graphs(e);
+    for (auto& t : e->trk) t.release();
     e->classes.clear();
+    for (auto s : e->cst)
+        if (s) (void)hipStreamDestroy(s);
+    for (auto v : e->cev)
+        if (v) (void)hipEventDestroy(v);
+    if (e->fork) (void)hipEventDestroy(e->fork);
     if (e->hstage) (void)hipHostFree(e->hstage);
     delete[] e->ebuf;
     if (e->st) (void)hipStreamDestroy(e->st);
@@ -848,8 +1031,10 @@ extern "C" int mwf_ens_create(const mwf_params* members, int32_t n, mwf_ens** ou
     e->ebuf = new double[n];
     e->relam.assign(n, -1);
     e->trace.resize(n);
+    e->ftrace.resize(n);
     int rc = 0;
-    if (hipStreamCreate(&e->st) != hipSuccess) rc = fail(MWF_E_HIP, "hipStreamCreate failed");
+    if (hipStreamCreate(&e->st) != hipSuccess || hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess)
+        rc = fail(MWF_E_HIP, "hipStreamCreate / hipEventCreate failed");
     std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
     for (int i = 0; i < n; i++)
         groups[std::make_tuple(members[i].precision, members[i].k0, members[i].mm, members[i].nn)].push_back(i);
@@ -858,8 +1043,17 @@ extern "C" int mwf_ens_create(const mwf_params* members, int32_t n, mwf_ens** ou
         if (rc) break;
         std::vector<mwf_params> ps;
         for (int i : kv.second) ps.push_back(members[i]);
+        hipStream_t cs = nullptr;
+        hipEvent_t ce = nullptr;
+        if (hipStreamCreate(&cs) != hipSuccess) rc = fail(MWF_E_HIP, "hipStreamCreate failed");
+        e->cst.push_back(cs);
+        if (!rc && hipEventCreateWithFlags(&ce, hipEventDisableTiming) != hipSuccess)
+            rc = fail(MWF_E_HIP, "hipEventCreate failed");
+        e->cev.push_back(ce);
+        e->trk.emplace_back();
+        if (rc) break;
         std::unique_ptr<SizeClass> k(new SizeClass());
-        rc = k->init(ps, e->st);
+        rc = k->init(ps, cs);
         if (rc) break;
         for (size_t j = 0; j < kv.second.size(); j++) e->where[kv.second[j]] = {(int)e->classes.size(), (int)j};
         maxel = std::max(maxel, k->mpt_elems);
@@ -932,26 +1126,60 @@ extern "C" int mwf_ens_energy(mwf_ens* e, double* energies)
     return 0;
 }
 
-static int ens_eager_step(mwf_ens* e)
+// One tracking sample of class c on stream st: energies, optionally the
+// turbulent points (vel_mpt2phys without the laminar profile into the class's
+// own U and phys buffers, then k_turbcount), then k_track_push.
+static int ens_sample(mwf_ens* e, size_t c, hipStream_t st)
+{
+    SizeClass& k = *e->classes[c];
+    ClassTrack& t = e->trk[c];
+    RCCHK(k.energy(st));
+    if (e->turb_on) {
+        hipError_t r = k.plan.xinv(k.bt, k.nmem, k.g0, st, k.c, t.Ud, nullptr, k.twMM(), k.twPd, 0);
+        if (r == hipSuccess) r = k.plan.zphys(k.bt, k.nmem, k.g0, st, t.Ud, t.physd, k.twNN(), k.twQ());
+        if (r != hipSuccess) return fail(MWF_E_HIP, std::string("turbulence sample: ") + hipGetErrorString(r));
+        const int npts = ((k.g0.P + e->turb_xs - 1) / e->turb_xs) * ((k.g0.Q + e->turb_zs - 1) / e->turb_zs);
+        hipLaunchKernelGGL(k_turbcount, dim3((npts + 255) / 256, k.nmem), dim3(256), 0, st, t.physd, k.geos,
+                           e->turb_xs, e->turb_zs, e->turb_ct, t.count);
+        HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_track_push, dim3(1), dim3(256), 0, st, k.nmem, k.edev, e->turb_on ? t.count : nullptr,
+                       turb_den(k.g0, e->turb_xs, e->turb_zs), e->track_thr, (long long)e->track_every, t.etrace,
+                       t.ftrace, t.fill, t.total, t.relam, mwf_ens::CAP);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// nsteps steps of every class (and a tracking sample after them when
+// sample is set): forked from e->st onto the class streams, joined back.
+static int ens_chunk(mwf_ens* e, int nsteps, bool sample)
 {
     auto plain = [&](int, auto&& f) {
         hipError_t r = f();
         return r == hipSuccess ? 0 : fail(MWF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(r));
     };
-    for (auto& k : e->classes) RCCHK(k->step(e->st, plain));
+    HIPCHK(hipEventRecord(e->fork, e->st));
+    for (size_t c = 0; c < e->classes.size(); c++) {
+        hipStream_t cs = e->cst[c];
+        HIPCHK(hipStreamWaitEvent(cs, e->fork, 0));
+        for (int s = 0; s < nsteps; s++) RCCHK(e->classes[c]->step(cs, plain));
+        if (sample) RCCHK(ens_sample(e, c, cs));
+        HIPCHK(hipEventRecord(e->cev[c], cs));
+        HIPCHK(hipStreamWaitEvent(e->st, e->cev[c], 0));
+    }
     return 0;
 }
 
-static int ens_graph(mwf_ens* e, int steps, hipGraphExec_t* exec)
+static int ens_graph(mwf_ens* e, int steps, bool sample, hipGraphExec_t* exec)
 {
-    auto it = e->graphs.find(steps);
+    const auto key = std::make_pair(steps, (int)sample);
+    auto it = e->graphs.find(key);
     if (it != e->graphs.end()) {
         *exec = it->second.second;
         return 0;
     }
     HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int s = 0; s < steps && !rc; s++) rc = ens_eager_step(e);
+    int rc = ens_chunk(e, steps, sample);
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(e->st, &g);
     if (rc) {
@@ -965,45 +1193,81 @@ static int ens_graph(mwf_ens* e, int steps, hipGraphExec_t* exec)
         (void)hipGraphDestroy(g);
         return fail(MWF_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
     }
-    e->graphs[steps] = {g, x};
+    e->graphs[key] = {g, x};
     *exec = x;
     return 0;
 }
 
-static int ens_track(mwf_ens* e)
+// Move the device trace buffers to the host vectors and empty them.
+static int ens_drain(mwf_ens* e)
 {
-    RCCHK(ens_energies(e));
-    for (size_t i = 0; i < e->where.size(); i++) {
-        e->trace[i].push_back(e->ebuf[i]);
-        if (e->relam[i] < 0 && e->ebuf[i] <= e->track_thr) e->relam[i] = e->step_count;
+    if (e->track_every <= 0) return 0;
+    HIPCHK(hipStreamSynchronize(e->st));
+    std::vector<double> eb, fb;
+    std::vector<long long> rl;
+    for (size_t c = 0; c < e->classes.size(); c++) {
+        SizeClass& k = *e->classes[c];
+        ClassTrack& t = e->trk[c];
+        int fill = 0;
+        HIPCHK(hipMemcpy(&fill, t.fill, sizeof(int), hipMemcpyDeviceToHost));
+        fill = std::min(fill, (int)mwf_ens::CAP);
+        eb.resize((size_t)fill * k.nmem);
+        rl.resize(k.nmem);
+        if (fill) HIPCHK(hipMemcpy(eb.data(), t.etrace, sizeof(double) * eb.size(), hipMemcpyDeviceToHost));
+        if (fill && e->turb_on) {
+            fb.resize(eb.size());
+            HIPCHK(hipMemcpy(fb.data(), t.ftrace, sizeof(double) * fb.size(), hipMemcpyDeviceToHost));
+        }
+        HIPCHK(hipMemcpy(rl.data(), t.relam, sizeof(long long) * k.nmem, hipMemcpyDeviceToHost));
+        const int zero = 0;
+        HIPCHK(hipMemcpy(t.fill, &zero, sizeof(int), hipMemcpyHostToDevice));
+        for (size_t i = 0; i < e->where.size(); i++) {
+            if (e->where[i].first != (int)c) continue;
+            const int j = e->where[i].second;
+            for (int r = 0; r < fill; r++) {
+                e->trace[i].push_back(eb[(size_t)r * k.nmem + j]);
+                if (e->turb_on) e->ftrace[i].push_back(fb[(size_t)r * k.nmem + j]);
+            }
+            e->relam[i] = rl[j];
+        }
     }
+    e->pending = 0;
     return 0;
 }
 
-// nsteps of every member; steps after the first are replayed from a graph of
-// GRAPH_STEPS steps (energies recorded between replays when tracking is on).
+// nsteps of every member; steps after the first are replayed from graphs.
+// With tracking on, a chunk runs up to the next multiple of the tracking
+// interval and ends with a sample, all inside the graph.
 extern "C" int mwf_ens_step(mwf_ens* e, int32_t nsteps)
 {
     if (!e) return fail(MWF_E_ARG, "null handle");
     if (nsteps < 0) return fail(MWF_E_ARG, "nsteps < 0");
+    const int every = e->track_every;
     int done = 0;
-    if (e->track_every > 0 && e->step_count == 0 && e->trace[0].empty()) RCCHK(ens_track(e));
     while (done < nsteps) {
-        if (e->nst_all) { // first step: forward Euler, outside the graph
-            RCCHK(ens_eager_step(e));
+        int chunk;
+        bool sample = false;
+        if (every > 0) {
+            chunk = std::min(every - (int)(e->step_count % every), nsteps - done);
+            sample = (e->step_count + chunk) % every == 0;
+        } else {
+            chunk = std::min(50, nsteps - done);
+        }
+        if (sample && e->pending + 1 > mwf_ens::CAP) RCCHK(ens_drain(e));
+        if (e->nst_all) { // first step (forward Euler) eagerly, outside the graphs
+            RCCHK(ens_chunk(e, 1, sample && chunk == 1));
             e->nst_all = 0;
+            if (sample && chunk == 1) e->pending++;
             done++;
             e->step_count++;
-        } else {
-            int chunk = e->track_every > 0 ? e->track_every - (int)(e->step_count % e->track_every) : 10;
-            chunk = std::min(chunk, nsteps - done);
-            hipGraphExec_t x = nullptr;
-            RCCHK(ens_graph(e, chunk, &x));
-            HIPCHK(hipGraphLaunch(x, e->st));
-            done += chunk;
-            e->step_count += chunk;
+            continue;
         }
-        if (e->track_every > 0 && e->step_count % e->track_every == 0) RCCHK(ens_track(e));
+        hipGraphExec_t x = nullptr;
+        RCCHK(ens_graph(e, chunk, sample, &x));
+        HIPCHK(hipGraphLaunch(x, e->st));
+        if (sample) e->pending++;
+        done += chunk;
+        e->step_count += chunk;
     }
     return 0;
 }
@@ -1015,25 +1279,96 @@ extern "C" int mwf_ens_sync(mwf_ens* e)
     return 0;
 }
 
+// (Re)allocate and reset every class's tracking buffers for the current
+// settings; the first sample (the state as it is now) is taken at once.
+static int ens_track_reset(mwf_ens* e)
+{
+    HIPCHK(hipStreamSynchronize(e->st));
+    ens_drop_graphs(e); // the graphs hold the old sample kernels and buffers
+    for (auto& v : e->trace) v.clear();
+    for (auto& v : e->ftrace) v.clear();
+    std::fill(e->relam.begin(), e->relam.end(), (int64_t)-1);
+    e->pending = 0;
+    e->step_count = 0;
+    for (size_t c = 0; c < e->classes.size(); c++) {
+        SizeClass& k = *e->classes[c];
+        ClassTrack& t = e->trk[c];
+        t.release();
+        if (e->track_every <= 0) continue;
+        const size_t n = k.nmem;
+        HIPCHK(hipMalloc((void**)&t.etrace, sizeof(double) * n * mwf_ens::CAP));
+        HIPCHK(hipMalloc((void**)&t.ftrace, sizeof(double) * n * mwf_ens::CAP));
+        HIPCHK(hipMalloc((void**)&t.fill, sizeof(int)));
+        HIPCHK(hipMalloc((void**)&t.total, sizeof(long long)));
+        HIPCHK(hipMalloc((void**)&t.relam, sizeof(long long) * n));
+        HIPCHK(hipMalloc((void**)&t.count, sizeof(unsigned long long) * n));
+        HIPCHK(hipMemset(t.fill, 0, sizeof(int)));
+        HIPCHK(hipMemset(t.total, 0, sizeof(long long)));
+        HIPCHK(hipMemset(t.count, 0, sizeof(unsigned long long) * n));
+        std::vector<long long> neg(n, -1);
+        HIPCHK(hipMemcpy(t.relam, neg.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+        if (e->turb_on) {
+            const size_t rsz = k.prec == MWF_MIXED ? sizeof(float2) : sizeof(double2);
+            HIPCHK(hipMalloc(&t.Ud, rsz * (size_t)k.g0.P * 2 * k.g0.KK * k.g0.NN * n));
+            HIPCHK(hipMalloc((void**)&t.physd, sizeof(double) * (size_t)k.g0.KK * k.g0.Q * k.g0.P * n));
+        }
+    }
+    if (e->track_every <= 0) return 0;
+    // the sample of the current state (step 0 of the tracked run)
+    HIPCHK(hipEventRecord(e->fork, e->st));
+    for (size_t c = 0; c < e->classes.size(); c++) {
+        HIPCHK(hipStreamWaitEvent(e->cst[c], e->fork, 0));
+        RCCHK(ens_sample(e, c, e->cst[c]));
+        HIPCHK(hipEventRecord(e->cev[c], e->cst[c]));
+        HIPCHK(hipStreamWaitEvent(e->st, e->cev[c], 0));
+    }
+    e->pending = 1;
+    return 0;
+}
+
 extern "C" int mwf_ens_track(mwf_ens* e, int32_t every, double threshold)
 {
     if (!e || every < 0) return fail(MWF_E_ARG, "bad arguments");
     e->track_every = every;
     e->track_thr = threshold;
-    for (auto& t : e->trace) t.clear();
-    std::fill(e->relam.begin(), e->relam.end(), (int64_t)-1);
-    return 0;
+    if (every == 0) e->turb_on = 0;
+    return ens_track_reset(e);
+}
+
+extern "C" int mwf_ens_track_turb(mwf_ens* e, int32_t xs, int32_t zs, double ct)
+{
+    if (!e || xs < 1 || zs < 1) return fail(MWF_E_ARG, "bad arguments");
+    if (e->track_every <= 0) return fail(MWF_E_STATE, "enable tracking with mwf_ens_track first");
+    for (auto& k : e->classes)
+        if (xs > k->g0.P || zs > k->g0.Q) return fail(MWF_E_ARG, "strides exceed a member's grid");
+    e->turb_on = 1;
+    e->turb_xs = xs;
+    e->turb_zs = zs;
+    e->turb_ct = ct;
+    return ens_track_reset(e);
 }
 
 extern "C" int mwf_ens_trace(mwf_ens* e, int32_t member, double* energies, int32_t cap, int32_t* count,
                              int64_t* relam_step)
 {
     if (!e || !count || !relam_step || member < 0 || member >= (int)e->where.size()) return fail(MWF_E_ARG, "bad arguments");
+    RCCHK(ens_drain(e));
     const auto& t = e->trace[member];
     const int n = (int)std::min<size_t>(t.size(), cap > 0 ? (size_t)cap : 0);
     if (energies && n > 0) std::memcpy(energies, t.data(), sizeof(double) * n);
     *count = (int32_t)t.size();
     *relam_step = e->relam[member];
+    return 0;
+}
+
+extern "C" int mwf_ens_trace_turb(mwf_ens* e, int32_t member, double* fractions, int32_t cap, int32_t* count)
+{
+    if (!e || !count || member < 0 || member >= (int)e->where.size()) return fail(MWF_E_ARG, "bad arguments");
+    RCCHK(ens_drain(e));
+    const auto& t = e->ftrace[member];
+    const int n = (int)std::min<size_t>(t.size(), cap > 0 ? (size_t)cap : 0);
+    if (fractions && n > 0) std::memcpy(fractions, t.data(), sizeof(double) * n);
+    *count = (int32_t)t.size();
     return 0;
 }
 

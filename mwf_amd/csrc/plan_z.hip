@@ -3,6 +3,8 @@
 // plans.hpp.
 #include "kernels.hpp"
 #include "plans.hpp"
+#include "transform.hpp"
+#include "zflux.hpp"
 
 #if !defined(MWF_R) || !defined(MWF_K0)
 #error "compile with -DMWF_R=double|float -DMWF_K0=4|6"
@@ -49,15 +51,43 @@ static hipError_t launch_zb(const Batch& bt, int nmem, const Geo& g, hipStream_t
 }
 
 template <typename R, int K0, int LOGNN>
-static hipError_t launch_zphys(const Batch& bt, const Geo& g, hipStream_t s, const void* U, double* phys,
+static hipError_t launch_zphys(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, double* phys,
                                const void* twNN, const void* twQ)
 {
     using CT = cplx_t<R>;
     constexpr int TZ = zphys_threads<LOGNN>();
     constexpr int XB = TZ >> LOGNN;
-    dim3 grid((g.P + XB - 1) / XB);
+    dim3 grid((g.P + XB - 1) / XB, nmem);
     hipLaunchKernelGGL((k_zphys<R, K0, LOGNN>), grid, dim3(TZ), (zphys_lds<R, LOGNN>()), s, bt, (const CT*)U, phys,
                        (const CT*)twNN, (const CT*)twQ);
+    return hipGetLastError();
+}
+
+template <typename R, int LOGNN>
+static hipError_t launch_zfwdp(const Geo& g, hipStream_t s, const double* phys, void* NL, const void* twNN,
+                               const void* twQ)
+{
+    using CT = cplx_t<R>;
+    constexpr int XB = zfp_threads<LOGNN>() >> LOGNN;
+    hipLaunchKernelGGL((k_zfwd_phys<R, LOGNN>), dim3((g.P + XB - 1) / XB), dim3(zfp_threads<LOGNN>()),
+                       (zfp_lds<R, LOGNN>()), s, g, phys, (CT*)NL, (const CT*)twNN, (const CT*)twQ);
+    return hipGetLastError();
+}
+
+// flux-form z kernel: two pairs per batch
+constexpr int ZC_ZP = 2;
+// W: waves per SIMD the kernel is compiled for: 4 (two 512-thread blocks per
+// CU, 128 VGPRs) or 2 (one block, 256 VGPRs)
+template <typename R, int K0, int LOGNN, int W>
+static hipError_t launch_zc(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
+                            const void* twNN, const void* twQ)
+{
+    using CT = cplx_t<R>;
+    constexpr int XB = zc_threads<LOGNN>() >> LOGNN;
+    dim3 grid((g.P + XB - 1) / XB, 1, nmem);
+    hipLaunchKernelGGL((k_zc<R, K0, LOGNN, ZC_ZP, W>), grid, dim3(zc_threads<LOGNN>()),
+                       (zc_lds_bytes<R, LOGNN, ZC_ZP>()), s, bt, (const CT*)U, (CT*)NL, (const CT*)twNN,
+                       (const CT*)twQ);
     return hipGetLastError();
 }
 
@@ -69,6 +99,9 @@ template <typename R, int K0> void pick_z(int ln, Plan& p)
         p.zf = launch_z<R, K0, L>;              \
         p.zf_alt2 = launch_z2<R, K0, L>;        \
         p.zphys = launch_zphys<R, K0, L>;       \
+        p.zfwdp = launch_zfwdp<R, L>;           \
+        p.zc = launch_zc<R, K0, L, 4>;          \
+        p.zc_alt = launch_zc<R, K0, L, 2>;      \
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
@@ -80,6 +113,7 @@ template <typename R, int K0> void pick_z(int ln, Plan& p)
     case 10: // one x-row per 1024-thread block: the three fold phases in sequence
         p.zf = launch_zb<R, K0, 10>;
         p.zphys = launch_zphys<R, K0, 10>;
+        p.zfwdp = launch_zfwdp<R, 10>;
         break;
     default: break;
     }
@@ -93,6 +127,9 @@ template <typename R, int K0> void prepare_z(int ln)
         set_lds(k_z<R, K0, L>, z_lds_bytes<R, K0, L>());          \
         set_lds(k_z2<R, K0, L>, z2_lds_bytes<R, L>());            \
         set_lds(k_zphys<R, K0, L>, zphys_lds<R, L>());            \
+        set_lds(k_zfwd_phys<R, L>, zfp_lds<R, L>());              \
+        set_lds(k_zc<R, K0, L, ZC_ZP, 4>, zc_lds_bytes<R, L, ZC_ZP>()); \
+        set_lds(k_zc<R, K0, L, ZC_ZP, 2>, zc_lds_bytes<R, L, ZC_ZP>()); \
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
@@ -104,6 +141,7 @@ template <typename R, int K0> void prepare_z(int ln)
     case 10:
         set_lds(k_zb<R, K0, 10>, zb_lds_bytes<R, 10>());
         set_lds(k_zphys<R, K0, 10>, zphys_lds<R, 10>());
+        set_lds(k_zfwd_phys<R, 10>, zfp_lds<R, 10>());
         break;
     default: break;
     }

@@ -14,7 +14,9 @@ struct Plan {
     hipError_t (*xfwd)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, const void*,
                        const double2*, int, int) = nullptr;
     hipError_t (*zf)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*, const void*) = nullptr;
-    hipError_t (*zphys)(const Batch&, const Geo&, hipStream_t, const void*, double*, const void*, const void*) = nullptr;
+    // z inverse to the phys layout, members back to back
+    hipError_t (*zphys)(const Batch&, int, const Geo&, hipStream_t, const void*, double*, const void*,
+                        const void*) = nullptr;
     // fused forward + update + next inverse
     hipError_t (*xstep)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
                         const void*, const double2*, int, int) = nullptr;
@@ -24,6 +26,22 @@ struct Plan {
                          const void*) = nullptr;
     hipError_t (*zf_alt2)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*,
                           const void*) = nullptr;
+    // flux form (zflux.hpp): z kernel and the x kernels that read its B/A lines
+    // and write U without x-derivative lines
+    hipError_t (*zc)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*, const void*) = nullptr;
+    hipError_t (*zc_alt)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*,
+                         const void*) = nullptr; // the flux z kernel at 2 waves per SIMD instead of 4
+    hipError_t (*xinv_f)(const Batch&, int, const Geo&, hipStream_t, const double2*, void*, double*, const void*,
+                         const double2*, int) = nullptr;
+    hipError_t (*xfwd_f)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, const void*,
+                         const double2*, int, int) = nullptr;
+    hipError_t (*xstep_f)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
+                          const void*, const double2*, int, int) = nullptr;
+    // standalone transforms of the transform module (transform.hpp):
+    // spec -> U lines, phys -> NL, NL -> spec
+    hipError_t (*sxinv)(const Geo&, hipStream_t, const double2*, void*, const void*, const double2*) = nullptr;
+    hipError_t (*zfwdp)(const Geo&, hipStream_t, const double*, void*, const void*, const void*) = nullptr;
+    hipError_t (*sxfwd)(const Geo&, hipStream_t, const void*, double2*, const void*, const double2*) = nullptr;
 };
 
 // x kernels depend on the precision and MM only, z kernels on the precision,

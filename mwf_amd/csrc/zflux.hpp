@@ -1,0 +1,203 @@
+// zflux.hpp -- the z pass of the flux (conservative) form of the nonlinear
+// term.
+//
+// For the divergence-free Galerkin field, (u.grad)u = div(u u), and the
+// wall-normal derivative moves onto the bases exactly (gen_galerkin.py has
+// the derivation).  The z pass then needs only the KK base lines of U (u_k,
+// v_k, w_k; no x- or z-derivatives): per point it forms the 6 K0 - 3
+// projected products uu_i, uw_i, ww_i, uv_i, vw_i, vv_i (flux_gen.hpp),
+// transforms them forward, and writes per spec slot s a line B_s (to be
+// differentiated in x by the x kernel) and a line A_s = (ak1 terms) + i gamma
+// n (z terms).  NL thus holds 2 KK lines per x-row (B at line s, A at KK + s),
+// U only KK.  Compared with the advective z kernel this transforms
+// ceil(KK/2) + ceil(NR/2) pairs instead of ceil(3 KK/2) + ceil(KK/2) (M4: 17
+// against 23) and keeps KK values per point instead of 2 KK.
+//
+// Block: T = max(512, NN) threads, XB = T/NN x-rows; thread (xl, r) owns the
+// points z = 3r + b, b = 0..2, of row xl, and the output index n = r of that
+// row in every line, so a line whose rows straddle two batches is summed in
+// the thread's registers.  Sequences of ZP pairs per batch, transformed in
+// place in LDS (ip_fft), which keeps the block at or under 80 KB so that two
+// blocks share a CU at NN = 512 in FP64.
+#pragma once
+#include "kernels.hpp"
+#include "flux_gen.hpp"
+
+template <int LOGNN> constexpr int zc_threads() { return (1 << LOGNN) > ZT ? (1 << LOGNN) : ZT; }
+
+// in-place transform buffer, then the pass twiddle table of the NN-point
+// transforms and the first NN entries of exp(2 pi i j/Q) (twq2 extends them)
+template <typename R, int LOGNN, int ZP> constexpr size_t zc_lds_bytes()
+{
+    constexpr int NN = 1 << LOGNN, XB = zc_threads<LOGNN>() / NN;
+    return sizeof(cplx_t<R>) * ((size_t)XB * ZP * 3 * NN + ptw_size<NN>() + (size_t)NN);
+}
+
+template <typename R, int K0, int LOGNN, int ZP, int WAVES>
+__global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, const cplx_t<R>* __restrict__ U,
+                                                                   cplx_t<R>* __restrict__ NL,
+                                                                   const cplx_t<R>* __restrict__ twNN,
+                                                                   const cplx_t<R>* __restrict__ twQ)
+{
+    using CT = cplx_t<R>;
+    using FX = Flux<K0>;
+    const Geo g = bt.geo[blockIdx.z];
+    U += blockIdx.z * bt.u_stride;
+    NL += blockIdx.z * bt.nl_stride;
+    constexpr int KK = 3 * K0 - 1;
+    constexpr int NN = 1 << LOGNN;
+    constexpr int T = zc_threads<LOGNN>();
+    constexpr int XB = T / NN;
+    constexpr int H = NN / 2;
+    constexpr int NPI = (KK + 1) / 2;                 // inverse pairs: base lines 2p, 2p + 1
+    constexpr int NBI = (NPI + ZP - 1) / ZP;
+    constexpr int NR = FX::NR;
+    constexpr int NPO = (NR + 1) / 2;                 // forward pairs: rows order(2p), order(2p + 1)
+    constexpr int NBO = (NPO + ZP - 1) / ZP;
+    constexpr int SEQ = XB * ZP * 3;
+    constexpr int FT = XB * ZP * H / T;               // fold tasks per thread
+    static_assert(FT * T == XB * ZP * H, "fold tasks must tile the block");
+    extern __shared__ unsigned char smem_raw[];
+    CT* buf = reinterpret_cast<CT*>(smem_raw);        // [SEQ][NN]
+    CT* twNl = buf + SEQ * NN;                        // [ptw_size<NN>()] pass twiddles
+    CT* twQl = twNl + ptw_size<NN>();                 // [NN]: w^k, k < NN
+
+    const int tid = threadIdx.x;
+    const int xl = tid >> LOGNN, r = tid & (NN - 1);
+    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * XB;
+    const int x = x0 + xl;
+    const bool valid = x < g.P;
+    const bool wide = g.NTx * (int)sizeof(CT) >= 64;
+    // with one x-row per block the sequences of the valid pairs of a batch
+    // come first, so a part-padding batch transforms only those
+    auto nseq = [](int nv) { return XB == 1 ? 3 * nv : SEQ; };
+
+    // fold task f of this thread: tau = tid + T f -> (row fx, pair fp, index fr);
+    // its four values of the next batch are loaded while this one transforms
+    CT pf[FT][4];
+    auto prefetch = [&](int batch) {
+#pragma unroll
+        for (int f = 0; f < FT; f++) {
+            const int tau = tid + T * f;
+            const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
+            const int xg = x0 + fx;
+            const int pr = batch * ZP + fp;
+            const int r2 = fr == 0 ? H : NN - fr;
+            const CT z{(R)0, (R)0};
+            pf[f][0] = pf[f][1] = pf[f][2] = pf[f][3] = z;
+            if (xg < g.P && pr < NPI) {
+                pf[f][0] = U[uidx(g, xg, 2 * pr, fr)];
+                pf[f][1] = U[uidx(g, xg, 2 * pr, r2)];
+                if (2 * pr + 1 < KK) {
+                    pf[f][2] = U[uidx(g, xg, 2 * pr + 1, fr)];
+                    pf[f][3] = U[uidx(g, xg, 2 * pr + 1, r2)];
+                }
+            }
+        }
+    };
+
+    R vb[3][KK];
+    prefetch(0);
+    ptw_fill<CT, LOGNN>(twNl, twNN, tid, T);
+    for (int i = tid; i < NN; i += T) twQl[i] = twQ[i];
+    __syncthreads();
+
+    // ---- inverse z transforms of the base lines (c2r, +1) ----
+    static_for<0, NBI>([&](auto bi) {
+        constexpr int BN = decltype(bi)::value;
+        constexpr int NV = NPI - BN * ZP < ZP ? NPI - BN * ZP : ZP; // valid pairs
+#pragma unroll
+        for (int f = 0; f < FT; f++) {
+            const int tau = tid + T * f;
+            const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
+            if (XB == 1 && fp >= NV) continue;
+            const int fm = fr == 0 ? H : NN - fr;
+            CT* seq = buf + ((fx * ZP + fp) * 3) * NN;
+            const CT one{(R)1, (R)0};
+            zfold_bw<R>(seq, 0, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], one, one);
+            zfold_bw<R>(seq + NN, 1, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twQl[fr], twQl[fm]);
+            zfold_bw<R>(seq + 2 * NN, 2, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twq2<R>(twQl, NN, 2 * fr),
+                        twq2<R>(twQl, NN, 2 * fm));
+        }
+        if constexpr (BN + 1 < NBI) prefetch(BN + 1);
+        __syncthreads();
+        ip_fft<CT, LOGNN, +1, T, SEQ, true>(buf, nseq(NV), twNl);
+        static_for<0, NV>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+            constexpr int s1 = 2 * (BN * ZP + p);
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const CT zv = buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
+                vb[b][s1] = zv.x;
+                if constexpr (s1 + 1 < KK) vb[b][s1 + 1] = zv.y;
+            }
+        });
+        __syncthreads();
+    });
+
+    // ---- products, forward z transforms (r2c, -1), line assembly ----
+    const R scale = (R)g.scale;
+    const R gn = (R)(g.gamma * r);       // i gamma n at the thread's output n = r
+    const R beta = (R)g.beta;
+    CT acc[2 * KK];
+    static_for<0, NBO>([&](auto ri) {
+        constexpr int RD = decltype(ri)::value;
+        constexpr int NV = NPO - RD * ZP < ZP ? NPO - RD * ZP : ZP; // valid output pairs
+        static_for<0, NV>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+            constexpr int q1 = 2 * (RD * ZP + p);
+            constexpr int r1 = FX::order(q1), r2 = FX::order(q1 + 1);
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                R v1 = (R)0, v2 = (R)0;
+                if constexpr (r1 >= 0) v1 = FX::template row<r1>(vb[b]);
+                if constexpr (r2 >= 0) v2 = FX::template row<r2>(vb[b]);
+                buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)] = CT{v1, v2};
+            }
+        });
+        __syncthreads();
+        ip_fft<CT, LOGNN, -1, T, SEQ, true>(buf, nseq(NV), twNl);
+        static_for<0, NV>([&](auto pi) {
+            constexpr int p = decltype(pi)::value;
+            constexpr int q1 = 2 * (RD * ZP + p);
+            const int rm = (NN - r) & (NN - 1);
+            CT X[2] = {CT{(R)0, (R)0}, CT{(R)0, (R)0}};
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const CT zn = buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
+                const CT zm = buf[((xl * ZP + p) * 3 + b) * NN + pidx(rm)];
+                const CT F1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
+                const CT F2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
+                if (b == 0) {
+                    X[0] = F1;
+                    X[1] = F2;
+                } else {
+                    const CT w = cconj(twq2<R>(twQl, NN, r * b));
+                    X[0] = cadd(X[0], cmul(F1, w));
+                    X[1] = cadd(X[1], cmul(F2, w));
+                }
+            }
+            static_for<0, 2>([&](auto hi) {
+                constexpr int q = q1 + decltype(hi)::value;
+                constexpr int row = FX::order(q);
+                if constexpr (row >= 0) {
+                    const CT xs = cscale(X[decltype(hi)::value], scale);
+                    static_for<0, FX::nuse(row)>([&](auto ui) {
+                        constexpr int u = decltype(ui)::value;
+                        constexpr int L = FX::use_line(row, u), kind = FX::use_kind(row, u);
+                        constexpr int sg = FX::use_sign(row, u);
+                        CT term = xs;
+                        if constexpr (kind == 1) term = cscale(xs, (R)sg * beta);
+                        else if constexpr (kind == 2) term = CT{-(xs.y * gn), xs.x * gn};
+                        if constexpr (FX::first(L) == q) acc[L] = term;
+                        else acc[L] = cadd(acc[L], term);
+                        if constexpr (FX::last(L) == q) {
+                            if (valid) st_piece(&NL[nidx(g, x, L, r)], acc[L], wide);
+                        }
+                    });
+                }
+            });
+        });
+        __syncthreads();
+    });
+}

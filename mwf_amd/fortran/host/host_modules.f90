@@ -45,6 +45,10 @@ module variables
       double precision :: Re(i_K, 0:i_M1, 0:i_Np-1)
       double precision :: Im(i_K, 0:i_M1, 0:i_Np-1)
    end type mpt
+   type spec
+      double precision :: Re(i_KK, 0:i_M1, 0:i_Np-1)
+      double precision :: Im(i_KK, 0:i_M1, 0:i_Np-1)
+   end type spec
 contains
    subroutine var_mpt_init(a)
       type(mpt), intent(out) :: a

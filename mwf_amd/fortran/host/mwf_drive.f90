@@ -7,7 +7,9 @@
 ! to final_state.bin (Re then Im, raw doubles) after vel_sync_host, and the
 ! device diagnostics of the final state to diag.bin (raw doubles): energy,
 ! spectrum maxima m_, n_, the history at y = 0.25, the turb sample grid, then
-! the AB2 history vel_onl (Re, Im) and nst.
+! the AB2 history vel_onl (Re, Im) and nst.  Last, the transform variant:
+! tra_spec2phys of the spec s(k,m,n) = (sin(k + 3m + 7n), cos(2k - m + 5n))
+! and tra_phys2spec of that phys, to transform.bin (phys, then spec Re, Im).
 !   usage: mwf_drive <nsteps> <seed> <rms> <k0>
 !
 program mwf_drive
@@ -15,9 +17,12 @@ program mwf_drive
    use parameters
    use variables
    use velocity
+   use transform
    use mwf_capi
    implicit none
-   type(phys) :: vp
+   type(phys) :: vp, tp
+   type(spec) :: ts, ts2
+   integer :: k, m, n
    type(mwf_params) :: p
    double precision :: E, scl, k0
    double precision :: m_(0:i_MM-1), n_(0:i_NN-1), H(4, i_H), T(3, 0:i_SN-1, 0:i_SMp-1)
@@ -32,6 +37,7 @@ program mwf_drive
 
    tim_t = 0d0
    tim_step = 0
+   call tra_precompute()
    call vel_precompute()
    p%k0 = i_K0; p%mm = i_MM; p%nn = i_NN; p%dragall = 1; p%precision = MWF_FP64; p%reserved = 0
    p%alpha = d_alpha; p%gamma = d_gamma; p%re = d_Re; p%dt = d_dt
@@ -62,5 +68,18 @@ program mwf_drive
    call vel_sync_onl_host()
    open(newunit=uo, file='diag.bin', access='stream', form='unformatted', status='replace')
    write(uo) E, m_, n_, H, T, vel_onl%Re, vel_onl%Im, merge(1d0, 0d0, nst)
+   close(uo)
+   do n = 0, i_Np - 1
+      do m = 0, i_M1
+         do k = 1, i_KK
+            ts%Re(k, m, n) = dsin(dble(k + 3 * m + 7 * n))
+            ts%Im(k, m, n) = dcos(dble(2 * k - m + 5 * n))
+         end do
+      end do
+   end do
+   call tra_spec2phys(ts, tp)
+   call tra_phys2spec(tp, ts2)
+   open(newunit=uo, file='transform.bin', access='stream', form='unformatted', status='replace')
+   write(uo) tp%Re, ts2%Re, ts2%Im
    close(uo)
 end program mwf_drive

@@ -41,6 +41,26 @@ module mwf_capi
          real(c_double), intent(out) :: re(*), im(*)
       end function mwf_get_state
 
+      integer(c_int) function mwf_put_state(h, re, im) bind(C, name='mwf_put_state')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: re(*), im(*)
+      end function mwf_put_state
+
+      integer(c_int) function mwf_spec2phys(h, re, im, phys) bind(C, name='mwf_spec2phys')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: re(*), im(*)
+         real(c_double), intent(out) :: phys(*)
+      end function mwf_spec2phys
+
+      integer(c_int) function mwf_phys2spec(h, phys, re, im) bind(C, name='mwf_phys2spec')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: phys(*)
+         real(c_double), intent(out) :: re(*), im(*)
+      end function mwf_phys2spec
+
       integer(c_int) function mwf_step(h, nsteps) bind(C, name='mwf_step')
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: h

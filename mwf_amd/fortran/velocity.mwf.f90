@@ -38,6 +38,7 @@ module velocity
    type(c_ptr), private :: mwf_h = c_null_ptr
    logical, private :: host_newer = .true.   ! host vel_c changed since the last upload
    logical, private :: dev_newer = .false.   ! device state is ahead of host vel_c
+   logical, private :: restart_ab2 = .true.  ! the next upload starts AB2 afresh (nst)
 
 contains
 
@@ -68,14 +69,26 @@ contains
       call mwf_check(mwf_create(p, mwf_h), 'mwf_create')
       host_newer = .true.
       dev_newer = .false.
+      restart_ab2 = .true.
    end subroutine vel_precompute
+
+   ! upload host vel_c: the first upload after vel_precompute is the loaded
+   ! state and starts AB2 with forward Euler (nst, program/velocity.f90:24-27,
+   ! 48); later ones (vel_imposesym's projections) replace vel_c and keep the
+   ! AB2 history, as the reference's in-place projections do
+   subroutine upload_state()
+      if (restart_ab2) then
+         call mwf_check(mwf_set_state(mwf_h, vel_c%Re, vel_c%Im), 'mwf_set_state')
+         restart_ab2 = .false.
+      else
+         call mwf_check(mwf_put_state(mwf_h, vel_c%Re, vel_c%Im), 'mwf_put_state')
+      end if
+      host_newer = .false.
+   end subroutine upload_state
 
    ! one step of program/velocity.f90:21-41 on the GPU
    subroutine vel_TS()
-      if (host_newer) then
-         call mwf_check(mwf_set_state(mwf_h, vel_c%Re, vel_c%Im), 'mwf_set_state')
-         host_newer = .false.
-      end if
+      if (host_newer) call upload_state()
       call mwf_check(mwf_step(mwf_h, 1_c_int32_t), 'mwf_step')
       dev_newer = .true.
       nst = .false.
@@ -90,9 +103,7 @@ contains
 
    ! the device holds the newest state: upload vel_c if the host changed it
    subroutine vel_sync_device()
-      if (.not. host_newer) return
-      call mwf_check(mwf_set_state(mwf_h, vel_c%Re, vel_c%Im), 'mwf_set_state')
-      host_newer = .false.
+      if (host_newer) call upload_state()
    end subroutine vel_sync_device
 
    ! AB2 history for a warm restart (program/velocity.f90:16,24-27): fills
@@ -111,6 +122,7 @@ contains
       call mwf_check(mwf_set_onl(mwf_h, vel_onl%Re, vel_onl%Im), 'mwf_set_onl')
       host_newer = .false.
       dev_newer = .false.
+      restart_ab2 = .false.
       nst = .false.
    end subroutine vel_load_onl
 
@@ -147,7 +159,8 @@ contains
       call mwf_check(mwf_turb_sample(mwf_h, int(i_DM, c_int32_t), int(i_DN, c_int32_t), t), 'mwf_turb_sample')
    end subroutine vel_turb_sample_dev
 
-   ! symmetry projections act on the host copy (program/velocity.f90:68-74)
+   ! symmetry projections act on the host copy (program/velocity.f90:68-74);
+   ! the next upload replaces vel_c on the device and keeps the AB2 history
    subroutine vel_imposesym()
       if (.not. (s_reflect .or. s_uvreflect .or. s_wreflect)) return
       call vel_sync_host()

@@ -54,7 +54,7 @@ def lib():
         L.orc_create.argtypes = [ctypes.POINTER(OrcParams)]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_set_threads.argtypes = [ctypes.c_int]
-        for name in ("orc_set_state", "orc_get_state", "orc_nonlinear", "orc_lhsrhs",
+        for name in ("orc_set_state", "orc_put_state", "orc_get_state", "orc_nonlinear", "orc_lhsrhs",
                      "orc_addlam"):
             getattr(L, name).argtypes = [ctypes.c_void_p, _dp, _dp][: 2 if name == "orc_addlam" else 3]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -131,6 +131,11 @@ class Oracle:
     def set_state(self, re, im):
         re, im = _f64(re).reshape(self.mpt_shape), _f64(im).reshape(self.mpt_shape)
         lib().orc_set_state(self._h, _p(re), _p(im))
+
+    def put_state(self, re, im):
+        """Replace the state keeping the AB2 history (vel_imposesym)."""
+        re, im = _f64(re).reshape(self.mpt_shape), _f64(im).reshape(self.mpt_shape)
+        lib().orc_put_state(self._h, _p(re), _p(im))
 
     def get_state(self):
         re, im = np.empty(self.mpt_shape), np.empty(self.mpt_shape)

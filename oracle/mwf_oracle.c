@@ -463,6 +463,15 @@ void orc_set_state(orc *o, const double *re, const double *im)
     o->nst = 1; /* vel_precompute: program/velocity.f90:48 */
 }
 
+/* vel_imposesym (program/velocity.f90:68-74) changes vel_c between steps and
+ * leaves vel_onl and nst as they are */
+void orc_put_state(orc *o, const double *re, const double *im)
+{
+    size_t nm = (size_t)o->K * o->M * o->NN;
+    memcpy(o->c_re, re, nm * sizeof(double));
+    memcpy(o->c_im, im, nm * sizeof(double));
+}
+
 void orc_get_state(const orc *o, double *re, double *im)
 {
     size_t nm = (size_t)o->K * o->M * o->NN;

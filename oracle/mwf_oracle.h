@@ -35,6 +35,7 @@ void orc_destroy(orc *o);
 void orc_set_threads(int n);
 
 void orc_set_state(orc *o, const double *re, const double *im);
+void orc_put_state(orc *o, const double *re, const double *im);
 void orc_get_state(const orc *o, double *re, double *im);
 void orc_step(orc *o, int nsteps);
 double orc_energy(orc *o);

@@ -1,10 +1,12 @@
-"""The Fortran host side: the GPU variant of module velocity.
+"""The Fortran host side: the GPU variants of module velocity and module
+transform.
 
-CPU: it compiles against the reference's own parameters/variables/modes
+CPU: they compile against the reference's own parameters/variables/modes
 modules (when oracle/_ref has them) and against the host harness.
 GPU: the driver with main.f90's loop order runs the M4 64^2 case for 100
 steps; vel_energy.dat (printed '(2e20.12)') and the final state match the
-oracle.
+oracle, and the transform variant's tra_spec2phys / tra_phys2spec match the
+oracle's transforms.
 """
 import math
 import os
@@ -26,7 +28,7 @@ def _have(exe):
 
 @pytest.mark.skipif(not _have("amdflang"), reason="amdflang absent")
 def test_variant_compiles_against_host_harness(tmp_path):
-    for src in ("host/host_modules.f90", "mwf_capi.f90", "velocity.mwf.f90"):
+    for src in ("host/host_modules.f90", "mwf_capi.f90", "velocity.mwf.f90", "transform.mwf.f90"):
         subprocess.run(["amdflang", "-c", os.path.join(FDIR, src), "-J", str(tmp_path)], cwd=tmp_path, check=True)
 
 
@@ -34,9 +36,10 @@ def test_variant_compiles_against_host_harness(tmp_path):
                     reason="reference modules not built here")
 def test_variant_compiles_against_reference_modules(tmp_path):
     subprocess.run(["amdflang", "-c", os.path.join(FDIR, "mwf_capi.f90"), "-J", str(tmp_path)], cwd=tmp_path, check=True)
-    subprocess.run(["amdflang", "-c", os.path.join(FDIR, "velocity.mwf.f90"), "-I", os.path.join(ROOT, "oracle", "_ref"),
-                    "-J", str(tmp_path)], cwd=tmp_path, check=True)
-    assert os.path.exists(tmp_path / "velocity.mod")
+    for src in ("velocity.mwf.f90", "transform.mwf.f90"):
+        subprocess.run(["amdflang", "-c", os.path.join(FDIR, src), "-I", os.path.join(ROOT, "oracle", "_ref"),
+                        "-J", str(tmp_path)], cwd=tmp_path, check=True)
+    assert os.path.exists(tmp_path / "velocity.mod") and os.path.exists(tmp_path / "transform.mod")
 
 
 @pytest.mark.gpu
@@ -98,3 +101,18 @@ def test_fortran_driver_matches_oracle(tmp_path):
     assert herr <= 1e-12, herr
     assert parts[7][0] == 0.0 and not first
     print("[fortran] device diagnostics and vel_onl match the Python handle")
+
+    # the transform variant (tra_spec2phys, tra_phys2spec) against the oracle
+    KK, P, Q = 11, 96, 96
+    t = np.fromfile(tmp_path / "transform.bin", dtype=np.float64)
+    assert t.size == KK * Q * P + 2 * KK * M * NN
+    tp = t[:KK * Q * P].reshape(P, Q, KK)
+    sre = t[KK * Q * P:KK * Q * P + KK * M * NN].reshape(NN, M, KK)
+    sim = t[KK * Q * P + KK * M * NN:].reshape(NN, M, KK)
+    n_, m_, k_ = np.meshgrid(np.arange(NN), np.arange(M), np.arange(1, KK + 1), indexing="ij")
+    ire, iim = np.sin(k_ + 3.0 * m_ + 7.0 * n_), np.cos(2.0 * k_ - m_ + 5.0 * n_)
+    want = o.spec2phys(ire, iim)
+    assert np.abs(tp - want).max() <= 1e-13 * np.abs(want).max()
+    wr, wi = o.phys2spec(want)
+    assert np.abs(sre - wr).max() <= 1e-13 * np.abs(wr).max() and np.abs(sim - wi).max() <= 1e-13 * np.abs(wi).max()
+    print("[fortran] transform variant matches the oracle")

@@ -170,3 +170,47 @@ def test_blend_is_the_interpstate_combination():
     for g, x, y in zip(got, s1, s2):
         assert np.array_equal(g, (1.0 - d) * x + d * y)
     assert c.get_onl()[2], "the blended state starts with a forward-Euler step"
+
+
+def oracle_turb(o, re, im, xs, zs):
+    """epos (program/velocity.f90:115-126, the oracle's reference-order C
+    restatement) of vel_mpt2phys(c) on the strided grid of io_writeVTK_txz."""
+    ph = o.mpt2phys(re, im)
+    return np.array([[oracle.epos(o.k0, ph[ix, iz]) for iz in range(0, o.Q, zs)] for ix in range(0, o.P, xs)])
+
+
+def safe_threshold(ep, q):
+    """A threshold near the q-quantile of the sampled epos values, halfway
+    between two neighbours, so that rounding cannot move a point across it."""
+    v = np.sort(ep.ravel())
+    i = int(q * (v.size - 1))
+    return 0.5 * (v[i] + v[i + 1])
+
+
+@pytest.mark.parametrize("k0,mm,nn,xs,zs", [(4, 32, 32, 6, 6), (6, 64, 32, 1, 1), (4, 128, 256, 7, 5),
+                                            (4, 512, 512, 6, 6)])
+def test_turb_fraction_matches_oracle(k0, mm, nn, xs, zs):
+    """io_writeVTK_txz's count (program/io.f90:594-615): epos > ct on the strided
+    grid, over (N_+1)(M_+1) with M_ = 3M/xs, N_ = 3N/zs."""
+    p, v, o = setup(k0, mm, nn)
+    re, im = v.get_state()
+    ep = oracle_turb(o, re, im, xs, zs)
+    den = (o.Q // zs + 1) * (o.P // xs + 1)
+    for q in (0.1, 0.5, 0.9):
+        ct = safe_threshold(ep, q)
+        want = np.count_nonzero(ep > ct) / den
+        got = v.turb_fraction(xs, zs, ct)
+        print(f"[turbfrac] k0={k0} mm={mm} nn={nn} q={q}: {got:.6f} vs {want:.6f}", flush=True)
+        assert got == want
+    assert v.turb_fraction(xs, zs, 1e300) == 0.0
+    # the diagnostic does not disturb the stepping
+    a = mw.Velocity(p)
+    a.set_state(re, im)
+    v.set_state(re, im)
+    v.turb_fraction(xs, zs, 0.0)
+    a.vel_TS(2)
+    v.vel_TS(2)
+    x, y = a.get_state(), v.get_state()
+    assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+    v.close()
+    a.close()

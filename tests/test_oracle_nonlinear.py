@@ -183,3 +183,80 @@ def test_oracle_two_steps_follow_pinned_nonlinear():
     got = o.get_state()
     for part in range(2):
         np.testing.assert_allclose(got[part], c2[part], rtol=0, atol=1e-13 * np.abs(c2[part]).max())
+
+
+# ---------------------------------------------------------------- flux form
+
+def _gen():
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mwf_amd", "csrc")
+    sys.path.insert(0, here)
+    try:
+        import gen_galerkin
+    finally:
+        sys.path.pop(0)
+    return gen_galerkin
+
+
+def flux_composition(o, re, im):
+    """vel_nonlinear through the flux form of the generated tables (flux_gen.hpp):
+    rows of the base values on the 3/2 grid, forward transforms, then
+    N_s = i alpha m B_s + A_s with A_s = sum of (s beta) row and i gamma n row."""
+    G = _gen()
+    k0, kk = o.k0, o.KK
+    sr, si = o.mpt2spec(re, im)
+    s = (sr + 1j * si).copy()
+    s[0, 0, 1] += math.cos(o.params.theta)        # vel_addlam, program/velocity.f90:76-82
+    s[0, 0, 2 * k0] += math.sin(o.params.theta)
+    p = _to_xz(o, s)
+    rows = G.flux_rows(k0)
+    R = np.zeros(p.shape[:2] + (len(rows),))
+    for r, (_, _, terms) in enumerate(rows):
+        for a, b, c in terms:
+            R[:, :, r] += float(c) * p[:, :, a] * p[:, :, b]
+    Rs = _from_xz(o, R)
+    kx, kz = _wavenumbers(o)
+    out = np.zeros(s.shape, complex)
+    for L, terms in enumerate(G.flux_lines(k0)):
+        acc = 0
+        for r, kind, sg in terms:
+            f = {0: 1.0, 1: sg * BETA, 2: 1j * kz[:, :, 0]}[kind]
+            acc = acc + f * Rs[:, :, r]
+        if L < kk:
+            out[:, :, L] += 1j * kx[:, :, 0] * acc
+        else:
+            out[:, :, L - kk] += acc
+    return out
+
+
+@pytest.mark.parametrize("k0,mm,nn,theta,alpha,gamma", CASES)
+def test_flux_form_equals_advective_form(k0, mm, nn, theta, alpha, gamma):
+    """The generated flux-form rows and line assembly reproduce the advective
+    nonlinear term of the independent 3-D evaluation, so a flux-form kernel
+    computes vel_nonlinear up to rounding."""
+    o = oracle.Oracle(k0, mm, nn, alpha, gamma, 200.0, 0.02, theta=theta)
+    re, im = _state(o, 5 * k0 + mm + nn)
+    adv = composed_3d(o, re, im, theta)[0]
+    flux = flux_composition(o, re, im)
+    assert np.abs(flux - adv).max() <= 1e-13 * np.abs(adv).max()
+
+
+@pytest.mark.parametrize("k0", [4, 6])
+def test_flux_rows_are_exact_projections(k0):
+    """Each generated row equals the quadrature projection of its product."""
+    G = _gen()
+    rng = np.random.default_rng(k0)
+    y = np.arange(NY) * (4.0 / NY)
+    h, _, g, _ = _ybases(k0, y)
+    su, sv, sw = slice(0, k0), slice(k0, 2 * k0 - 1), slice(2 * k0 - 1, 3 * k0 - 1)
+    for _ in range(10):
+        v = rng.standard_normal(3 * k0 - 1)
+        U, V, W = v[su] @ h, v[sv] @ g, v[sw] @ h
+        fields = {"uu": (U * U, h, 0), "uw": (U * W, h, 0), "ww": (W * W, h, 0),
+                  "uv": (U * V, g, 1), "vw": (V * W, g, 1), "vv": (V * V, h, 0)}
+        for name, i, terms in G.flux_rows(k0):
+            f, B, off = fields[name]
+            want = (f * B[i - off]).mean() / (B[i - off] ** 2).mean()
+            got = sum(float(c) * v[a] * v[b] for a, b, c in terms)
+            assert abs(got - want) <= 1e-13, (name, i)
