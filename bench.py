@@ -4,7 +4,8 @@
 
 Workload: the M4 basis, 1024 x 1024 Fourier modes (i_MM = i_NN = 512, a
 1536 x 1536 padded grid), Lx = Lz = 128 pi, Re = 180, dt = 0.02, seeded
-Gaussian-spectrum initial condition (seed 2, rms 0.3, k0 = 8), FP64.  A step
+Gaussian-spectrum initial condition (seed 2, scl 0.3 applied as the rms
+perturbation velocity, k0 = 8), FP64.  A step
 is one vel_TS (program/velocity.f90:21-41): nonlinear term through the
 3/2-dealiased transforms, AB2 and the implicit update.  The state is resident
 in HBM for the whole timed region.  The same workload is then run in the
@@ -31,6 +32,21 @@ METRIC = "steps/sec FP64 vs mixed at M4 1024²; ensemble member-steps/sec (1 MI3
 PEAK_HBM_GBS = 8000.0
 WORKLOAD = dict(k0=4, mm=512, nn=512, lx=128 * np.pi, lz=128 * np.pi, re=180.0, dt=0.02)
 IC = dict(seed=2, scl=0.3, k0=8.0)
+# turbulent-point threshold on epos for the ensembles' turbulent fractions
+# (utils/turbfrac.f90 reads it from input): 1e-2, 2 % of the laminar energy 0.5
+TURB_CT = 1e-2
+
+
+def step_bytes(mw, p):
+    """Algorithmic HBM bytes of one step of one member (the two-pass model):
+    state c read and written (complex FP64), AB2 history read and written (in
+    the transform precision), and one HBM round trip of the x-physical /
+    z-spectral intermediates: 3 KK lines of P x NN complex values each way
+    (written once, read once)."""
+    d = mw.dims(p)
+    si = 8 if p.precision == mw.MWF_MIXED else 16
+    mpt = d["K"] * d["M"] * p.nn
+    return 2 * 16 * mpt + 2 * si * mpt + 2 * 3 * d["KK"] * d["P"] * p.nn * si
 
 
 def run_gpu(mw, precision, steps, warmup, ic, workload=WORKLOAD):
@@ -77,6 +93,7 @@ def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20, track_every=50):
     for i in range(256):
         ens.set_state(i, *ics[i])
     ens.track(track_every, 1e-6 * 0.5)
+    ens.track_turb(6, 6, TURB_CT)
     ens.sync()
     t0 = time.perf_counter()
     ens.step(nsteps)
@@ -84,12 +101,14 @@ def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20, track_every=50):
     t1 = time.perf_counter()
     batched = 256 * nsteps / (t1 - t0)
     e = ens.energies()
-    relam = []
+    relam, tfrac = [], []
     for i in range(256):
         trace, rstep = ens.trace(i)
         relam.append(rstep)
-    relam = np.array(relam)
+        tfrac.append(ens.trace_turb(i)[-1])
+    relam, tfrac = np.array(relam), np.array(tfrac)
     ens.close()
+    bpm = step_bytes(mw, ps[0])
     # sequential on the GPU: one model per member, one after another
     vs = []
     for i in range(seq_members):
@@ -108,12 +127,18 @@ def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20, track_every=50):
     for v in vs:
         v.close()
     done = relam[relam >= 0]
-    return {"workload": "256 x M4 128x128 modes (MM=NN=64), Lx=Lz=16pi, Re U[150,220], rms U[0.05,0.4], "
+    return {"workload": "256 x M4 128x128 modes (MM=NN=64), Lx=Lz=16pi, Re U[150,220], scl U[0.05,0.4] (as rms velocity), "
                         "seeds 1000..1255, dt 0.02, FP64",
             "member_steps_per_s": batched, "steps": nsteps,
             "sequential_member_steps_per_s": seq, "sequential_sample": f"{seq_members} members x {seq_steps} steps",
             "speedup_vs_sequential": batched / seq, "finite": bool(np.all(np.isfinite(e))),
-            "graph": f"hipGraph replay of {track_every}-step chunks, energy traced every {track_every} steps",
+            "graph": f"hipGraph replay of {track_every}-step chunks with the energy and turbulent-fraction "
+                     f"samples inside the graph (device trace buffers)",
+            "bytes_per_member_step": bpm,
+            "roofline": {"bound": "hbm", "achieved_GBs": batched * bpm / 1e9, "peak": PEAK_HBM_GBS,
+                         "frac": batched * bpm / 1e9 / PEAK_HBM_GBS},
+            "turbulent_fraction_at_end": {"median": float(np.median(tfrac)), "min": float(tfrac.min()),
+                                          "max": float(tfrac.max()), "xs_zs": [6, 6], "ct": TURB_CT},
             "relaminarised": int(done.size), "turbulent_at_end": int(256 - done.size),
             "relam_time_median": float(np.median(done) * 0.02) if done.size else None,
             "relam_time_min": float(done.min() * 0.02) if done.size else None,
@@ -122,7 +147,7 @@ def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20, track_every=50):
 
 def config_m6_2048(mw, steps, err_steps, warmup=3):
     """M6 basis, 2048x2048 modes (MM=NN=1024, 3072^2 grid), Lx=Lz=256pi, Re=250,
-    dt=0.01, seed 3 (rms 0.3, k0 8): FP64 and mixed steps/s, then both forms
+    dt=0.01, seed 3 (scl 0.3 as rms velocity, k0 8): FP64 and mixed steps/s, then both forms
     stepped side by side from the same state with the per-step energy error
     and the state error of the mixed run against the FP64 run."""
     wl = dict(k0=6, mm=1024, nn=1024, lx=256 * np.pi, lz=256 * np.pi, re=250.0, dt=0.01)
@@ -149,7 +174,7 @@ def config_m6_2048(mw, steps, err_steps, warmup=3):
     v32.close()
     e_err = np.array(e_err)
     return {"workload": "M6 basis, 2048x2048 modes (MM=NN=1024, 3072^2 grid), Lx=Lz=256pi, Re=250, dt=0.01, "
-                        "seed 3 (rms 0.3, k0 8)",
+                        "seed 3 (scl 0.3 as rms velocity, k0 8)",
             "fp64_steps_per_s": f64["steps_per_s"], "mixed_steps_per_s": mix["steps_per_s"],
             "speedup_mixed": mix["steps_per_s"] / f64["steps_per_s"], "steps": steps,
             "fp64_kernels_ms": {n: round(k["avg_ms"], 4) for n, k in f64["kernels"].items()},
@@ -162,16 +187,16 @@ def config_m6_2048(mw, steps, err_steps, warmup=3):
 
 
 def config_m4_64(mw, steps):
-    """M4, 64x64 modes (96^2 grid), Lx=Lz=8pi, Re=200, dt=0.02, seed 1 (rms 0.1,
+    """M4, 64x64 modes (96^2 grid), Lx=Lz=8pi, Re=200, dt=0.02, seed 1 (scl 0.1 as rms,
     k0 4), FP64: the reference's CPU-sized case, steps/s on the GPU."""
     wl = dict(k0=4, mm=32, nn=32, lx=8 * np.pi, lz=8 * np.pi, re=200.0, dt=0.02)
     ic = mw.ic_gaussian(mw.make_params(**wl), 1, 0.1, 4.0)
     r = run_gpu(mw, mw.MWF_FP64, steps, 10, ic, wl)
-    return {"workload": "M4, 64x64 modes (96^2 grid), Lx=Lz=8pi, Re=200, dt=0.02, seed 1 (rms 0.1, k0 4), FP64",
+    return {"workload": "M4, 64x64 modes (96^2 grid), Lx=Lz=8pi, Re=200, dt=0.02, seed 1 (scl 0.1 as rms velocity, k0 4), FP64",
             "steps_per_s": r["steps_per_s"], "steps": steps, "finite": r["finite"], "energy": r["energy"]}
 
 
-def ensemble_cfg5(mw, nsteps, subset=32):
+def ensemble_cfg5(mw, nsteps, subset=32, track_every=50):
     """512 members, M4:M6 = 3:1, sizes {64,128,256,512}^2 modes, box 0.4 pi per
     mode, Re ~ U[150,300], seeds 5000..5511, dt 0.01, mixed form; energy error
     of a 32-member FP64 subset after the same steps."""
@@ -190,6 +215,8 @@ def ensemble_cfg5(mw, nsteps, subset=32):
     ens = mw.Ensemble(ps)
     for i in range(512):
         ens.set_state(i, *ics[i])
+    ens.track(track_every, 1e-6 * 0.5)
+    ens.track_turb(6, 6, TURB_CT)
     ens.step(1)
     ens.sync()
     t0 = time.perf_counter()
@@ -198,7 +225,10 @@ def ensemble_cfg5(mw, nsteps, subset=32):
     t1 = time.perf_counter()
     e_mixed = ens.energies()
     ncls = ens.nclasses()
+    relam = np.array([ens.trace(i)[1] for i in range(512)])
+    tfrac = np.array([ens.trace_turb(i)[-1] for i in range(512)])
     ens.close()
+    bytes_step = sum(step_bytes(mw, p) for p in ps)
     sub = list(range(subset))
     e64 = mw.Ensemble([ps64[i] for i in sub])
     for j, i in enumerate(sub):
@@ -210,28 +240,46 @@ def ensemble_cfg5(mw, nsteps, subset=32):
     return {"workload": "512 members, M4:M6 = 3:1, sizes {64,128,256,512}^2 modes, L = 0.4pi per mode, "
                         "Re U[150,300], seeds 5000..5511, dt 0.01, mixed FP32/FP64",
             "member_steps_per_s": 512 * nsteps / (t1 - t0), "steps": nsteps, "size_classes": ncls,
+            "bytes_per_member_step_mean": bytes_step / 512,
+            "roofline": {"bound": "hbm", "achieved_GBs": bytes_step * nsteps / (t1 - t0) / 1e9,
+                         "peak": PEAK_HBM_GBS, "frac": bytes_step * nsteps / (t1 - t0) / 1e9 / PEAK_HBM_GBS},
+            "graph": "size classes as parallel branches of one hipGraph per chunk; energy and turbulent-fraction "
+                     f"samples every {track_every} steps on the device",
+            "relaminarised": int((relam >= 0).sum()),
+            "turbulent_fraction_at_end": {"median": float(np.median(tfrac)), "min": float(tfrac.min()),
+                                          "max": float(tfrac.max()), "xs_zs": [6, 6], "ct": TURB_CT},
             "fp64_subset_energy_rel_err_max": float(err.max()),
             "fp64_subset_energy_rel_err_median": float(np.median(err)),
             "finite": bool(np.all(np.isfinite(e_mixed)))}
 
 
-def cpu_baseline(ic, nsteps):
-    import oracle
-    threads = min(os.cpu_count() or 1, 16)
-    oracle.set_threads(threads)
-    w = WORKLOAD
+def cpu_baseline(ic, nsteps, single_steps=1):
+    """The oracle (the repository's C restatement of the reference path, with
+    its own FFT: FFTW is absent offline) timed on the host on a bounded sample
+    of the same workload: nsteps steps on all host cores (at most 16 threads,
+    OpenMP over x/z lines), then single_steps on one core (the reference
+    itself is single-threaded per MPI rank)."""
     import math
+    import oracle
+    w = WORKLOAD
     o = oracle.Oracle(w["k0"], w["mm"], w["nn"], 2 * math.pi / w["lx"], 2 * math.pi / w["lz"],
                       w["re"], w["dt"])
+    threads = min(os.cpu_count() or 1, 16)
+    oracle.set_threads(threads)
     o.set_state(*ic)
     o.step(1)  # first step (forward Euler) untimed
     t0 = time.perf_counter()
     o.step(nsteps)
     t1 = time.perf_counter()
+    oracle.set_threads(1)
+    t2 = time.perf_counter()
+    o.step(single_steps)
+    t3 = time.perf_counter()
     return dict(value=nsteps / (t1 - t0), unit="steps/s", cores=threads, kind="port",
-                sample=f"{nsteps} steps of the same workload (M4 1024^2 modes, FP64) after one "
-                       f"untimed step; oracle = C restatement of the reference path with its own "
-                       f"FFT (FFTW 3.3.8 unavailable offline), OpenMP over x/z lines")
+                single_core_value=single_steps / (t3 - t2),
+                sample=f"{nsteps} steps of the same workload (M4 1024^2 modes, FP64) on {threads} threads after "
+                       f"one untimed step, then {single_steps} step(s) on 1 thread; oracle = C restatement of "
+                       f"the reference path with its own FFT (FFTW 3.3.8 unavailable offline)")
 
 
 def calib_copy(mib):
@@ -294,13 +342,13 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-steps", type=int, default=5000)
-    ap.add_argument("--m6-steps", type=int, default=100)
+    ap.add_argument("--m6-steps", type=int, default=500)
     ap.add_argument("--m6-err-steps", type=int, default=100)
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="rule-based kernel plans instead of timed ones")
     ap.add_argument("--plan", type=int, default=None,
                     help="force kernel plan bits (1 k_zb, 2 split x, 4 k_z2) instead of timing the plans")
-    ap.add_argument("--cfg5-steps", type=int, default=200)
+    ap.add_argument("--cfg5-steps", type=int, default=2000)
     ap.add_argument("--calib-copy", type=int, default=0, help="only run a MiB-sized device copy")
     ap.add_argument("--profile-case", choices=sorted(PROFILE_CASES), default=None,
                     help="only step one workload (FP64, then mixed) for --steps steps after --warmup, "
@@ -347,7 +395,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "M4 basis, 1024x1024 Fourier modes (MM=NN=512, 1536^2 padded grid), "
-                               "Lx=Lz=128pi, Re=180, dt=0.02, seeded Gaussian IC (seed 2, rms 0.3, k0 8)",
+                               "Lx=Lz=128pi, Re=180, dt=0.02, seeded Gaussian IC (seed 2, scl 0.3 as rms velocity, k0 8)",
                    "form": "FP64"},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,

@@ -214,7 +214,7 @@ template <int L, int Ns = 1> constexpr int npasses()
 // and writes A, ...  Ends with a barrier.  The result is in B when the pass
 // count is odd (fft_in_b<LOG>()), else in A.
 template <typename CT, int LOG, int SIGN, int T, bool PTW = false, int Ns = 1, int PASS = 0>
-__device__ __forceinline__ void pp_fft(CT* A, CT* B, int S, const CT* __restrict__ tw)
+__device__ __forceinline__ void pp_fft(CT* A, CT* B, int S, const CT* __restrict__ tw, int tid = threadIdx.x)
 {
     constexpr int L = 1 << LOG, LP = lpad(L);
     if constexpr (Ns < L) {
@@ -223,14 +223,14 @@ __device__ __forceinline__ void pp_fft(CT* A, CT* B, int S, const CT* __restrict
         const CT* src = (PASS & 1) ? B : A;
         CT* dst = (PASS & 1) ? A : B;
         const int total = S * NB;
-        for (int i = threadIdx.x; i < total; i += T) {
+        for (int i = tid; i < total; i += T) {
             const int s = i / NB, j = i % NB;
             CT x[R];
             bfly_load<CT, L, R, Ns, SIGN, PTW>(src + s * LP, j, x, tw);
             bfly_store<CT, R, Ns>(dst + s * LP, j, x);
         }
         __syncthreads();
-        pp_fft<CT, LOG, SIGN, T, PTW, Ns * R, PASS + 1>(A, B, S, tw);
+        pp_fft<CT, LOG, SIGN, T, PTW, Ns * R, PASS + 1>(A, B, S, tw, tid);
     }
 }
 template <int LOG> constexpr bool fft_in_b() { return (npasses<(1 << LOG)>() & 1) != 0; }

@@ -6,7 +6,9 @@
 //   k_xfwd : NL, c    -> c   forward x-DFT, spec2mpt, AB2 + implicit update
 //
 // Device layouts (complex values interleaved):
-//   c, onl   double2 [NN][K][ldM]      m in the reference order (m<0 at M+m)
+//   c        double2 [NN][K][ldM]      m in the reference order (m<0 at M+m)
+//   onl      cplx<R> [NN][K][ldM]      AB2 history in the precision of the
+//                                      transforms that produce it
 //   U        cplx<R> [P][2*KK][NN]     x-physical, z-spectral; lines 0..KK-1
 //                                      are u (spec slot order), KK..2KK-1 du/dx
 //   NL       cplx<R> [P][KK][NN]       x-physical, z-spectral nonlinear term
@@ -42,6 +44,18 @@ __device__ __forceinline__ void zwave(const Geo& g, int n, double& an1, double& 
     const double c = g.gamma * n;
     an1 = c;
     an2 = -(c * g.gamma * n);
+}
+
+// Phase b of the folded spectrum of the padded inverse x transform,
+// Y_b[r] = X[r] w^{rb} + X[r-MM] w^{(r-MM)b} with w1 = w^{rb}, w2 = w^{(MM-r)b}
+// (conjugated), in this operation order and without contraction: every
+// kernel that builds a U line (the fused step, its inverse-only form, k_xinv)
+// then produces the same bits, whatever each compilation would fuse.
+__device__ __forceinline__ double2 xfold_b(double2 X1, double2 X2, double2 w1, double2 w2)
+{
+#pragma clang fp contract(off)
+    return double2{(X1.x * w1.x - X1.y * w1.y) + (X2.x * w2.x + X2.y * w2.y),
+                   (X1.x * w1.y + X1.y * w1.x) + (X2.y * w2.x - X2.x * w2.y)};
 }
 
 // var_mpt2spec (program/variables.f90:390-434) for y-mode k at one (m, n):
@@ -195,9 +209,8 @@ __global__ __launch_bounds__(XT) void k_xinv(Batch bt, const double2* __restrict
             for (int b = 1; b < 3; b++) {
                 const double2 w1 = twP[r * b];
                 const double2 w2 = twP[(MM - r) * b]; // conj: exp(2 pi i (r-MM) b / P)
-                const double y_r = (X1.x * w1.x - X1.y * w1.y) + (X2.x * w2.x + X2.y * w2.y);
-                const double y_i = (X1.x * w1.y + X1.y * w1.x) + (X2.y * w2.x - X2.x * w2.y);
-                bt[b * LP + pidx(r)] = CT{(R)y_r, (R)y_i};
+                const double2 y = xfold_b(X1, X2, w1, w2);
+                bt[b * LP + pidx(r)] = CT{(R)y.x, (R)y.y};
             }
         }
         __syncthreads();
@@ -1112,6 +1125,28 @@ __device__ __forceinline__ void lhsrhs_k(const Geo& g, int k, int which, bool me
     }
 }
 
+// The AB2 history (the previous step's nonlinear term) is stored in the
+// precision of the transforms that produced it; the AB2 combination is always
+// evaluated in FP64.  In the mixed form this halves the history's bytes and
+// loses nothing: the stored values carry FP32 transform rounding already.
+__device__ __forceinline__ double2 hist_load(const double2* p) { return *p; }
+__device__ __forceinline__ double2 hist_load(const float2* p)
+{
+    const float2 v = *p;
+    return double2{(double)v.x, (double)v.y};
+}
+__device__ __forceinline__ void hist_store(double2* p, double2 v, bool stream = false)
+{
+    if (stream) st_stream(p, v);
+    else *p = v;
+}
+__device__ __forceinline__ void hist_store(float2* p, double2 v, bool stream = false)
+{
+    const float2 f = float2{(float)v.x, (float)v.y};
+    if (stream) st_stream(p, f);
+    else *p = f;
+}
+
 // vel_TS update (program/velocity.f90:28-35) in the reference's order
 __device__ __forceinline__ double2 ab2_update(double2 cv, double2 nl, double2 ov, double lhs, double rhs, double dt)
 {
@@ -1125,19 +1160,22 @@ __device__ __forceinline__ double2 ab2_update(double2 cv, double2 nl, double2 ov
 
 // ---------------------------------------------------------------------------
 // x forward + spec2mpt + update: block (n-tile, y-mode k).
-// mode 0: AB2/implicit update of c and onl (vel_TS); nst = first step.
-// mode 1: write the nonlinear term into onl only (vel_nonlinear).
+// mode 0: AB2/implicit update of c and the history onl (cplx<R>; vel_TS);
+//         nst = first step.
+// mode 1: write the nonlinear term into nlout only (double2; vel_nonlinear).
 // ---------------------------------------------------------------------------
 template <typename R, int LOGMM, bool FLUX>
 __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
-                                             double2* __restrict__ onl, const cplx_t<R>* __restrict__ twMM,
-                                             const double2* __restrict__ twP, int mode, int nst)
+                                             cplx_t<R>* __restrict__ onl, double2* __restrict__ nlout,
+                                             const cplx_t<R>* __restrict__ twMM, const double2* __restrict__ twP,
+                                             int mode, int nst)
 {
     using CT = cplx_t<R>;
     const Geo g = bt.geo[blockIdx.z];
     NL += blockIdx.z * bt.nl_stride;
     c += blockIdx.z * bt.c_stride;
     onl += blockIdx.z * bt.c_stride;
+    if (nlout) nlout += blockIdx.z * bt.c_stride;
     constexpr int MM = 1 << LOGMM;
     constexpr int LP = lpad(MM);
     constexpr int IPT = XTILE / XT;
@@ -1251,17 +1289,17 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
                 const size_t idx = ((size_t)n * g.K + slot) * g.ldM + mi;
                 const double2 nl = which == 0 ? psi[q][h] : phi[q][h];
                 if (mode == 1) {
-                    onl[idx] = nl;
+                    nlout[idx] = nl;
                     continue;
                 }
                 double lhs, rhs;
                 lhsrhs_k(g, k, which, mean, am2, an2, ak2, lhs, rhs);
                 const double2 cv = c[idx];
-                const double2 ov = nst ? nl : onl[idx];
+                const double2 ov = nst ? nl : hist_load(onl + idx);
                 double2 cn = ab2_update(cv, nl, ov, lhs, rhs, dt);
                 if (mean && which == 0 && k == 0) cn.x = 0.0;
                 c[idx] = cn;
-                onl[idx] = nl;
+                hist_store(onl + idx, nl);
             }
         }
     }
@@ -1282,7 +1320,7 @@ template <int LOGMM> constexpr int xs_of() { return 3 * (XTILE >> LOGMM) * group
 
 template <typename R, int LOGMM, bool INV_ONLY, bool FLUX>
 __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
-                                              double2* __restrict__ onl, cplx_t<R>* __restrict__ U,
+                                              cplx_t<R>* __restrict__ onl, cplx_t<R>* __restrict__ U,
                                               const cplx_t<R>* __restrict__ twMM, const double2* __restrict__ twP,
                                               int nst)
 {
@@ -1434,11 +1472,11 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                     double lhs, rhs;
                     lhsrhs_k(g, k, which, mean, am2, an2, ak2, lhs, rhs);
                     const double2 cv = c[idx];
-                    const double2 ov = nst ? nl : onl[idx];
+                    const double2 ov = nst ? nl : hist_load(onl + idx);
                     double2 cn = ab2_update(cv, nl, ov, lhs, rhs, dt);
                     if (mean && which == 0 && k == 0) cn.x = 0.0;
                     c[idx] = cn;
-                    st_stream(&onl[idx], nl);
+                    hist_store(onl + idx, nl, true);
                     // psi/phi now hold the new state, which the inverse half uses
                     if constexpr (XREG) {
                         if (which == 0) psi[q][h] = cn;
@@ -1500,9 +1538,8 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             for (int b = 1; b < 3; b++) {
                 const double2 w1 = twPl[r * b];
                 const double2 w2 = twPl[(MM - r) * b]; // conj: exp(2 pi i (r-MM) b / P)
-                const double y_r = (X[0].x * w1.x - X[0].y * w1.y) + (X[1].x * w2.x + X[1].y * w2.y);
-                const double y_i = (X[0].x * w1.y + X[0].y * w1.x) + (X[1].y * w2.x - X[1].x * w2.y);
-                bt3[b * MM + pidx(r)] = CT{(R)y_r, (R)y_i};
+                const double2 y = xfold_b(X[0], X[1], w1, w2);
+                bt3[b * MM + pidx(r)] = CT{(R)y.x, (R)y.y};
             }
         }
         __syncthreads();

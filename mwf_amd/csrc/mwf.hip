@@ -173,7 +173,9 @@ struct SizeClass {
     Batch bt{};
     Plan plan;
     size_t mpt_elems = 0;     // per member: NN * K * ldM
-    double2 *c = nullptr, *onl = nullptr, *cs = nullptr;
+    double2 *c = nullptr, *cs = nullptr;
+    void* onl = nullptr;      // AB2 history: double2 (FP64) or float2 (mixed) per element
+    size_t hsz = 0;           // bytes of one history element
     void *U = nullptr, *NL = nullptr;
     double *epart = nullptr, *edev = nullptr;
     int nepart = 0;           // per member
@@ -211,7 +213,8 @@ struct SizeClass {
         nepart = (g0.NN / g0.NTx) * g0.K0;
         RCCHK(dev_upload(&geos, geos_h.data(), geos_h.size()));
         HIPCHK(hipMalloc((void**)&c, sizeof(double2) * mpt_elems * nmem));
-        HIPCHK(hipMalloc((void**)&onl, sizeof(double2) * mpt_elems * nmem));
+        hsz = prec == MWF_MIXED ? sizeof(float2) : sizeof(double2);
+        HIPCHK(hipMalloc(&onl, hsz * mpt_elems * nmem));
         HIPCHK(hipMalloc((void**)&cs, sizeof(double2) * mpt_elems));
         HIPCHK(hipMalloc(&U, rsz * u_el * nmem));
         HIPCHK(hipMalloc(&NL, rsz * nl_el * nmem));
@@ -228,7 +231,7 @@ struct SizeClass {
         RCCHK(tf(&twNNf, g0.NN));
         RCCHK(tf(&twQf, g0.Q));
         HIPCHK(hipMemsetAsync(c, 0, sizeof(double2) * mpt_elems * nmem, st));
-        HIPCHK(hipMemsetAsync(onl, 0, sizeof(double2) * mpt_elems * nmem, st));
+        HIPCHK(hipMemsetAsync(onl, 0, hsz * mpt_elems * nmem, st));
         HIPCHK(hipStreamSynchronize(st));
         const int mode = g_tune.load();
         if (mode == 1) RCCHK(tune(st));
@@ -273,7 +276,7 @@ struct SizeClass {
     int reset_state(hipStream_t st)
     {
         HIPCHK(hipMemsetAsync(c, 0, sizeof(double2) * mpt_elems * nmem, st));
-        HIPCHK(hipMemsetAsync(onl, 0, sizeof(double2) * mpt_elems * nmem, st));
+        HIPCHK(hipMemsetAsync(onl, 0, hsz * mpt_elems * nmem, st));
         HIPCHK(hipStreamSynchronize(st));
         nst = 1;
         u_valid = false;
@@ -357,18 +360,21 @@ struct SizeClass {
             RCCHK(timed(4, [&] { return plan.xstep(bt, nmem, g0, st, NL, c, onl, U, twMM(), twPd, nst, 0); }));
             u_valid = true;
         } else {
-            RCCHK(timed(2, [&] { return plan.xfwd(bt, nmem, g0, st, NL, c, onl, twMM(), twPd, 0, nst); }));
+            RCCHK(timed(2, [&] { return plan.xfwd(bt, nmem, g0, st, NL, c, onl, nullptr, twMM(), twPd, 0, nst); }));
         }
         nst = 0;
         return 0;
     }
 
-    // energies of all members into edev[0..nmem)
-    int energy(hipStream_t st)
+    // energies of all members into out[0..nmem) (default edev), with the
+    // block partials in part (default epart, nepart * nmem doubles)
+    int energy(hipStream_t st, double* part = nullptr, double* out = nullptr)
     {
-        hipError_t e = plan.xinv(bt, nmem, g0, st, c, nullptr, epart, twMM(), twPd, 0);
+        if (!part) part = epart;
+        if (!out) out = edev;
+        hipError_t e = plan.xinv(bt, nmem, g0, st, c, nullptr, part, twMM(), twPd, 0);
         if (e != hipSuccess) return fail(MWF_E_HIP, std::string("energy launch: ") + hipGetErrorString(e));
-        hipLaunchKernelGGL(k_ereduce, dim3((nmem + 63) / 64), dim3(64), 0, st, epart, nepart, nmem, edev);
+        hipLaunchKernelGGL(k_ereduce, dim3((nmem + 63) / 64), dim3(64), 0, st, part, nepart, nmem, out);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -597,8 +603,8 @@ extern "C" int mwf_nonlinear(mwf_handle* h, double* nlre, double* nlim)
     k.u_valid = false;
     RCCHK(launch(k.plan.xinv(k.bt, 1, k.g0, h->st, k.c, k.U, nullptr, k.twMM(), k.twPd, 1)));
     RCCHK(launch(k.plan.zf(k.bt, 1, k.g0, h->st, k.U, k.NL, k.twNN(), k.twQ())));
-    // mode 1 writes the nonlinear term into its history argument (the scratch mpt)
-    RCCHK(launch(k.plan.xfwd(k.bt, 1, k.g0, h->st, k.NL, k.c, k.cs, k.twMM(), k.twPd, 1, 0)));
+    // mode 1 writes the nonlinear term into nlout (the scratch mpt) only
+    RCCHK(launch(k.plan.xfwd(k.bt, 1, k.g0, h->st, k.NL, k.c, k.onl, k.cs, k.twMM(), k.twPd, 1, 0)));
     return download_mpt(h, k.cs, nlre, nlim);
 }
 
@@ -723,18 +729,36 @@ extern "C" int mwf_phys2spec(mwf_handle* h, const double* phys, double* re, doub
     return 0;
 }
 
+// the history in the mixed form is FP32: widened on the way out, rounded on
+// the way in (what the mixed step itself stores)
 extern "C" int mwf_get_onl(mwf_handle* h, double* re, double* im, int32_t* first_step)
 {
     if (!h || !re || !im || !first_step) return fail(MWF_E_ARG, "null argument");
-    *first_step = h->cls->nst;
-    return download_mpt(h, h->cls->onl, re, im);
+    SizeClass& k = *h->cls;
+    *first_step = k.nst;
+    if (k.hsz == sizeof(double2)) return download_mpt(h, static_cast<double2*>(k.onl), re, im);
+    float2* f = reinterpret_cast<float2*>(h->hstage);
+    HIPCHK(hipMemcpyAsync(f, k.onl, sizeof(float2) * k.mpt_elems, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (size_t i = k.mpt_elems; i-- > 0;) h->hstage[i] = double2{(double)f[i].x, (double)f[i].y};
+    from_dev_layout(k.g0, h->hstage, re, im);
+    return 0;
 }
 
 extern "C" int mwf_set_onl(mwf_handle* h, const double* re, const double* im)
 {
     if (!h || !re || !im) return fail(MWF_E_ARG, "null argument");
-    RCCHK(upload_mpt(h, re, im, h->cls->onl));
-    h->cls->nst = 0;
+    SizeClass& k = *h->cls;
+    if (k.hsz == sizeof(double2)) {
+        RCCHK(upload_mpt(h, re, im, static_cast<double2*>(k.onl)));
+    } else {
+        to_dev_layout(k.g0, k.mpt_elems, re, im, h->hstage);
+        float2* f = reinterpret_cast<float2*>(h->hstage);
+        for (size_t i = 0; i < k.mpt_elems; i++) f[i] = float2{(float)h->hstage[i].x, (float)h->hstage[i].y};
+        HIPCHK(hipMemcpyAsync(k.onl, f, sizeof(float2) * k.mpt_elems, hipMemcpyHostToDevice, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+    }
+    k.nst = 0;
     return 0;
 }
 
@@ -891,13 +915,14 @@ static double kernel_bytes(const SizeClass& k, int kernel)
     const Geo& g = k.g0;
     const double s = k.prec == MWF_MIXED ? 8.0 : 16.0; // complex intermediate
     const double state = 16.0 * g.K * g.M * g.NN;      // one complex FP64 mpt
+    const double hist = s * g.K * g.M * g.NN;          // the AB2 history (precision of the transforms)
     const double uline = s * (double)g.P * g.NN;         // one x-physical line set
     switch (kernel) {
-    case KID_XINV: return state + 2.0 * g.KK * uline;        // read c, write u and du/dx
-    case KID_Z: return 2.0 * g.KK * uline + g.KK * uline;     // read U, write NL
-    case KID_XFWD: return g.KK * uline + 4.0 * state;        // read NL, c, onl; write c, onl
+    case KID_XINV: return state + g.UL * uline;               // read c, write U
+    case KID_Z: return (double)(g.UL + g.NLL) * uline;        // read U, write NL
+    case KID_XFWD: return g.NLL * uline + 2.0 * state + 2.0 * hist;          // read NL, c, onl; write c, onl
     case KID_ENERGY: return state;
-    case KID_XSTEP: return 3.0 * g.KK * uline + 4.0 * state; // read NL, c, onl; write c, onl, U
+    case KID_XSTEP: return (double)(g.NLL + g.UL) * uline + 2.0 * state + 2.0 * hist; // + write U
     default: return -1.0;
     }
 }

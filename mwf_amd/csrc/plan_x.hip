@@ -22,11 +22,12 @@ static hipError_t launch_xinv(const Batch& bt, int nmem, const Geo& g, hipStream
 
 template <typename R, int LOGMM, bool FLUX>
 static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
-                              double2* onl, const void* twMM, const double2* twP, int mode, int nst)
+                              void* onl, double2* nlout, const void* twMM, const double2* twP, int mode, int nst)
 {
     using CT = cplx_t<R>;
     dim3 grid(g.NN / g.NTx, g.K0, nmem);
-    hipLaunchKernelGGL((k_xfwd<R, LOGMM, FLUX>), grid, dim3(XT), (x_lds<R, LOGMM>()), s, bt, (const CT*)NL, c, onl,
+    hipLaunchKernelGGL((k_xfwd<R, LOGMM, FLUX>), grid, dim3(XT), (x_lds<R, LOGMM>()), s, bt, (const CT*)NL, c, (CT*)onl,
+                       nlout,
                        (const CT*)twMM, twP, mode, nst);
     return hipGetLastError();
 }
@@ -42,16 +43,16 @@ template <typename R, int LOGMM> static size_t xstep_lds()
 
 template <typename R, int LOGMM, bool FLUX>
 static hipError_t launch_xstep(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* NL, double2* c,
-                               double2* onl, void* U, const void* twMM, const double2* twP, int nst, int inv_only)
+                               void* onl, void* U, const void* twMM, const double2* twP, int nst, int inv_only)
 {
     using CT = cplx_t<R>;
     dim3 grid(g.NN / g.NTx, g.K0, nmem);
     if (inv_only)
         hipLaunchKernelGGL((k_xstep<R, LOGMM, true, FLUX>), grid, dim3(xs_of<LOGMM>()), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
-                           c, onl, (CT*)U, (const CT*)twMM, twP, nst);
+                           c, (CT*)onl, (CT*)U, (const CT*)twMM, twP, nst);
     else
         hipLaunchKernelGGL((k_xstep<R, LOGMM, false, FLUX>), grid, dim3(xs_of<LOGMM>()), (xstep_lds<R, LOGMM>()), s, bt, (const CT*)NL,
-                           c, onl, (CT*)U, (const CT*)twMM, twP, nst);
+                           c, (CT*)onl, (CT*)U, (const CT*)twMM, twP, nst);
     return hipGetLastError();
 }
 

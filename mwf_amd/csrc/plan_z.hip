@@ -77,7 +77,8 @@ static hipError_t launch_zfwdp(const Geo& g, hipStream_t s, const double* phys, 
 // flux-form z kernel: two pairs per batch
 constexpr int ZC_ZP = 2;
 // W: waves per SIMD the kernel is compiled for: 4 (two 512-thread blocks per
-// CU, 128 VGPRs) or 2 (one block, 256 VGPRs)
+// CU, 128 VGPRs, transforms in place) or 2 (one block, 256 VGPRs, ping-pong
+// transforms)
 template <typename R, int K0, int LOGNN, int W>
 static hipError_t launch_zc(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
                             const void* twNN, const void* twQ)
@@ -85,8 +86,8 @@ static hipError_t launch_zc(const Batch& bt, int nmem, const Geo& g, hipStream_t
     using CT = cplx_t<R>;
     constexpr int XB = zc_threads<LOGNN>() >> LOGNN;
     dim3 grid((g.P + XB - 1) / XB, 1, nmem);
-    hipLaunchKernelGGL((k_zc<R, K0, LOGNN, ZC_ZP, W>), grid, dim3(zc_threads<LOGNN>()),
-                       (zc_lds_bytes<R, LOGNN, ZC_ZP>()), s, bt, (const CT*)U, (CT*)NL, (const CT*)twNN,
+    hipLaunchKernelGGL((k_zc<R, K0, LOGNN, ZC_ZP, W, W == 2>), grid, dim3(zc_threads<LOGNN>()),
+                       (zc_lds_bytes<R, LOGNN, ZC_ZP, W == 2>()), s, bt, (const CT*)U, (CT*)NL, (const CT*)twNN,
                        (const CT*)twQ);
     return hipGetLastError();
 }
@@ -114,6 +115,9 @@ template <typename R, int K0> void pick_z(int ln, Plan& p)
         p.zf = launch_zb<R, K0, 10>;
         p.zphys = launch_zphys<R, K0, 10>;
         p.zfwdp = launch_zfwdp<R, 10>;
+        // the flux z kernel holds all three phases of a 3072-point row at
+        // 1024 threads (4 waves per SIMD) in FP32 only
+        if constexpr (std::is_same<R, float>::value) p.zc = launch_zc<R, K0, 10, 4>;
         break;
     default: break;
     }
@@ -128,8 +132,8 @@ template <typename R, int K0> void prepare_z(int ln)
         set_lds(k_z2<R, K0, L>, z2_lds_bytes<R, L>());            \
         set_lds(k_zphys<R, K0, L>, zphys_lds<R, L>());            \
         set_lds(k_zfwd_phys<R, L>, zfp_lds<R, L>());              \
-        set_lds(k_zc<R, K0, L, ZC_ZP, 4>, zc_lds_bytes<R, L, ZC_ZP>()); \
-        set_lds(k_zc<R, K0, L, ZC_ZP, 2>, zc_lds_bytes<R, L, ZC_ZP>()); \
+        set_lds(k_zc<R, K0, L, ZC_ZP, 4, false>, zc_lds_bytes<R, L, ZC_ZP, false>()); \
+        set_lds(k_zc<R, K0, L, ZC_ZP, 2, true>, zc_lds_bytes<R, L, ZC_ZP, true>());   \
         break;
         C_(4) C_(5) C_(6) C_(7) C_(8) C_(9)
 #undef C_
@@ -142,6 +146,8 @@ template <typename R, int K0> void prepare_z(int ln)
         set_lds(k_zb<R, K0, 10>, zb_lds_bytes<R, 10>());
         set_lds(k_zphys<R, K0, 10>, zphys_lds<R, 10>());
         set_lds(k_zfwd_phys<R, 10>, zfp_lds<R, 10>());
+        if constexpr (std::is_same<R, float>::value)
+            set_lds(k_zc<R, K0, 10, ZC_ZP, 4, false>, zc_lds_bytes<R, 10, ZC_ZP, false>());
         break;
     default: break;
     }

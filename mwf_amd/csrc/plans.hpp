@@ -11,14 +11,16 @@
 struct Plan {
     hipError_t (*xinv)(const Batch&, int, const Geo&, hipStream_t, const double2*, void*, double*, const void*,
                        const double2*, int) = nullptr;
-    hipError_t (*xfwd)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, const void*,
-                       const double2*, int, int) = nullptr;
+    // x forward + update: (NL, c, history onl (precision R), nonlinear-term
+    // output nlout (double2, mode 1 only), twiddles, mode, nst)
+    hipError_t (*xfwd)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, void*, double2*,
+                       const void*, const double2*, int, int) = nullptr;
     hipError_t (*zf)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*, const void*) = nullptr;
     // z inverse to the phys layout, members back to back
     hipError_t (*zphys)(const Batch&, int, const Geo&, hipStream_t, const void*, double*, const void*,
                         const void*) = nullptr;
     // fused forward + update + next inverse
-    hipError_t (*xstep)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
+    hipError_t (*xstep)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, void*, void*,
                         const void*, const double2*, int, int) = nullptr;
     // alternative z kernels the autotuner may pick: k_zb where k_z also runs,
     // and the two-points-per-thread k_z2
@@ -33,9 +35,9 @@ struct Plan {
                          const void*) = nullptr; // the flux z kernel at 2 waves per SIMD instead of 4
     hipError_t (*xinv_f)(const Batch&, int, const Geo&, hipStream_t, const double2*, void*, double*, const void*,
                          const double2*, int) = nullptr;
-    hipError_t (*xfwd_f)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, const void*,
-                         const double2*, int, int) = nullptr;
-    hipError_t (*xstep_f)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, double2*, void*,
+    hipError_t (*xfwd_f)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, void*, double2*,
+                         const void*, const double2*, int, int) = nullptr;
+    hipError_t (*xstep_f)(const Batch&, int, const Geo&, hipStream_t, const void*, double2*, void*, void*,
                           const void*, const double2*, int, int) = nullptr;
     // standalone transforms of the transform module (transform.hpp):
     // spec -> U lines, phys -> NL, NL -> spec

@@ -45,9 +45,8 @@ __global__ __launch_bounds__(XT) void k_sxinv(Geo g, const double2* __restrict__
         for (int b = 1; b < 3; b++) {
             const double2 w1 = twP[r * b];
             const double2 w2 = twP[(MM - r) * b]; // conj: exp(2 pi i (r-MM) b / P)
-            const double y_r = (X1.x * w1.x - X1.y * w1.y) + (X2.x * w2.x + X2.y * w2.y);
-            const double y_i = (X1.x * w1.y + X1.y * w1.x) + (X2.y * w2.x - X2.x * w2.y);
-            bt[b * LP + pidx(r)] = CT{(R)y_r, (R)y_i};
+            const double2 y = xfold_b(X1, X2, w1, w2);
+            bt[b * LP + pidx(r)] = CT{(R)y.x, (R)y.y};
         }
     }
     __syncthreads();

@@ -17,8 +17,9 @@
 // points z = 3r + b, b = 0..2, of row xl, and the output index n = r of that
 // row in every line, so a line whose rows straddle two batches is summed in
 // the thread's registers.  Sequences of ZP pairs per batch, transformed in
-// place in LDS (ip_fft), which keeps the block at or under 80 KB so that two
-// blocks share a CU at NN = 512 in FP64.
+// LDS either in place (ip_fft: one buffer, at or under 80 KB, so that two
+// blocks can share a CU) or ping-pong (pp_fft: two buffers, one barrier per
+// pass instead of two, for the form that runs one block per CU).
 #pragma once
 #include "kernels.hpp"
 #include "flux_gen.hpp"
@@ -27,13 +28,13 @@ template <int LOGNN> constexpr int zc_threads() { return (1 << LOGNN) > ZT ? (1 
 
 // in-place transform buffer, then the pass twiddle table of the NN-point
 // transforms and the first NN entries of exp(2 pi i j/Q) (twq2 extends them)
-template <typename R, int LOGNN, int ZP> constexpr size_t zc_lds_bytes()
+template <typename R, int LOGNN, int ZP, bool PP> constexpr size_t zc_lds_bytes()
 {
     constexpr int NN = 1 << LOGNN, XB = zc_threads<LOGNN>() / NN;
-    return sizeof(cplx_t<R>) * ((size_t)XB * ZP * 3 * NN + ptw_size<NN>() + (size_t)NN);
+    return sizeof(cplx_t<R>) * ((PP ? 2 : 1) * (size_t)XB * ZP * 3 * NN + ptw_size<NN>() + (size_t)NN);
 }
 
-template <typename R, int K0, int LOGNN, int ZP, int WAVES>
+template <typename R, int K0, int LOGNN, int ZP, int WAVES, bool PP>
 __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, const cplx_t<R>* __restrict__ U,
                                                                    cplx_t<R>* __restrict__ NL,
                                                                    const cplx_t<R>* __restrict__ twNN,
@@ -58,8 +59,17 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     constexpr int FT = XB * ZP * H / T;               // fold tasks per thread
     static_assert(FT * T == XB * ZP * H, "fold tasks must tile the block");
     extern __shared__ unsigned char smem_raw[];
-    CT* buf = reinterpret_cast<CT*>(smem_raw);        // [SEQ][NN]
-    CT* twNl = buf + SEQ * NN;                        // [ptw_size<NN>()] pass twiddles
+    CT* buf = reinterpret_cast<CT*>(smem_raw);        // [SEQ][NN]: input of each transform
+    CT* bufB = buf + (PP ? SEQ * NN : 0);             // [SEQ][NN]: the ping-pong partner
+    CT* twNl = bufB + SEQ * NN;                       // [ptw_size<NN>()] pass twiddles
+    // where a transform's result lies: in place, or after an odd number of
+    // ping-pong passes in the partner buffer
+    constexpr bool res_in_buf = !(PP && fft_in_b<LOGNN>());
+    CT* res = res_in_buf ? buf : bufB;
+    auto transform = [&](auto sign, int S, int t) {
+        if constexpr (PP) pp_fft<CT, LOGNN, decltype(sign)::value, T, true>(buf, bufB, S, twNl, t);
+        else ip_fft<CT, LOGNN, decltype(sign)::value, T, SEQ, true>(buf, S, twNl, t);
+    };
     CT* twQl = twNl + ptw_size<NN>();                 // [NN]: w^k, k < NN
 
     const int tid = threadIdx.x;
@@ -102,10 +112,21 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     for (int i = tid; i < NN; i += T) twQl[i] = twQ[i];
     __syncthreads();
 
+    // Thread-derived indices are recomputed per batch from an opaque copy of
+    // the thread index, so that the compiler does not hoist the LDS addresses
+    // of every batch's passes into registers next to the point values.
+    auto opaque_tid = [] {
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        return t;
+    };
+
     // ---- inverse z transforms of the base lines (c2r, +1) ----
     static_for<0, NBI>([&](auto bi) {
         constexpr int BN = decltype(bi)::value;
         constexpr int NV = NPI - BN * ZP < ZP ? NPI - BN * ZP : ZP; // valid pairs
+        const int tid = opaque_tid();
+        const int xl = tid >> LOGNN, r = tid & (NN - 1);
 #pragma unroll
         for (int f = 0; f < FT; f++) {
             const int tau = tid + T * f;
@@ -121,18 +142,20 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         }
         if constexpr (BN + 1 < NBI) prefetch(BN + 1);
         __syncthreads();
-        ip_fft<CT, LOGNN, +1, T, SEQ, true>(buf, nseq(NV), twNl);
+        transform(std::integral_constant<int, +1>{}, nseq(NV), tid);
         static_for<0, NV>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
             constexpr int s1 = 2 * (BN * ZP + p);
 #pragma unroll
             for (int b = 0; b < 3; b++) {
-                const CT zv = buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
+                const CT zv = res[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
                 vb[b][s1] = zv.x;
                 if constexpr (s1 + 1 < KK) vb[b][s1 + 1] = zv.y;
             }
         });
-        __syncthreads();
+        // the next batch writes buf: a barrier is needed only if the result
+        // was read from buf itself
+        if constexpr (res_in_buf) __syncthreads();
     });
 
     // ---- products, forward z transforms (r2c, -1), line assembly ----
@@ -143,6 +166,15 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     static_for<0, NBO>([&](auto ri) {
         constexpr int RD = decltype(ri)::value;
         constexpr int NV = NPO - RD * ZP < ZP ? NPO - RD * ZP : ZP; // valid output pairs
+        const int tid = opaque_tid();
+        const int xl = tid >> LOGNN, r = tid & (NN - 1);
+        // the base values pass through an empty asm per batch: products shared
+        // by rows of different batches are formed again in each batch instead
+        // of being computed once and held in registers across all of them
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int sl = 0; sl < KK; sl++) asm volatile("" : "+v"(vb[b][sl]));
         static_for<0, NV>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
             constexpr int q1 = 2 * (RD * ZP + p);
@@ -156,7 +188,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             }
         });
         __syncthreads();
-        ip_fft<CT, LOGNN, -1, T, SEQ, true>(buf, nseq(NV), twNl);
+        transform(std::integral_constant<int, -1>{}, nseq(NV), tid);
         static_for<0, NV>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
             constexpr int q1 = 2 * (RD * ZP + p);
@@ -164,8 +196,8 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             CT X[2] = {CT{(R)0, (R)0}, CT{(R)0, (R)0}};
 #pragma unroll
             for (int b = 0; b < 3; b++) {
-                const CT zn = buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
-                const CT zm = buf[((xl * ZP + p) * 3 + b) * NN + pidx(rm)];
+                const CT zn = res[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
+                const CT zm = res[((xl * ZP + p) * 3 + b) * NN + pidx(rm)];
                 const CT F1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
                 const CT F2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
                 if (b == 0) {
@@ -198,6 +230,6 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
                 }
             });
         });
-        __syncthreads();
+        if constexpr (res_in_buf) __syncthreads();
     });
 }

@@ -28,9 +28,13 @@ def setup(k0, mm, nn, steps=5, seed=3):
     return p, v, o
 
 
-@pytest.mark.parametrize("k0,mm,nn", [(4, 32, 32), (6, 64, 32), (4, 512, 512)])
-def test_warm_restart_is_bit_identical(k0, mm, nn):
-    p = mw.make_params(k0, mm, nn, mm * math.pi / 4, nn * math.pi / 4, 200.0, 0.02)
+@pytest.mark.parametrize("k0,mm,nn,prec", [(4, 32, 32, mw.MWF_FP64), (6, 64, 32, mw.MWF_FP64),
+                                            (4, 512, 512, mw.MWF_FP64), (4, 64, 64, mw.MWF_MIXED),
+                                            (6, 512, 256, mw.MWF_MIXED)])
+def test_warm_restart_is_bit_identical(k0, mm, nn, prec):
+    """The mixed form stores its AB2 history in FP32: get_onl widens it and
+    set_onl rounds it back, so the resume stays bit-identical there too."""
+    p = mw.make_params(k0, mm, nn, mm * math.pi / 4, nn * math.pi / 4, 200.0, 0.02, precision=prec)
     a = mw.Velocity(p)
     a.set_state(*mw.ic_gaussian(p, 1, 0.2, 4.0))
     assert a.get_onl()[2], "a new state starts with a forward-Euler step"
