@@ -83,6 +83,17 @@ int mwf_step(mwf_handle *h, int32_t nsteps);
  * state, reduced on the device (spectral Parseval sum, compensated). */
 int mwf_energy(mwf_handle *h, double *e);
 
+/* No counterpart in the reference: mwf_energy for callers that cannot
+ * allocate or synchronise (code under HIP graph capture, tight serving
+ * loops).  The perturbation energy of the current state is written to the
+ * device double e_dev, asynchronously on the handle's stream, with the
+ * per-block partials in a fixed 64 KB device array of the library (one per
+ * GPU, held until the process ends) instead of handle memory: no allocation
+ * and no synchronisation.  The array holds 8192 partials (any single-model
+ * handle fits: at most 3072); larger needs fail with MWF_E_ARG.  Calls that
+ * overlap on one GPU (handles on different streams) race on the array. */
+int mwf_energy_noalloc(mwf_handle *h, double *e_dev);
+
 /* vel_nonlinear (program/velocity.f90:92-113) of the current state, without
  * stepping: the mpt-form nonlinear term vel_nl. */
 int mwf_nonlinear(mwf_handle *h, double *nlre, double *nlim);

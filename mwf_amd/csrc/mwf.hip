@@ -482,10 +482,43 @@ struct Profiler {
 };
 
 // ---------------------------------------------------------------------------
+// Scratch of the entry point that allocates nothing (mwf_energy_noalloc): the
+// energy partials of one handle, in a file-scope device array that every GPU
+// the library runs on holds until the process ends (64 KB).  In host code the
+// name `scratch` is a host-side stand-in, not a device address: its device
+// address is looked up once per device, with that device current, and only
+// that address (g_scratch_dev) is passed to kernels.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_PARTS = 8192; // doubles
+static __device__ __align__(256) double scratch[NOALLOC_PARTS];
+constexpr int MAX_DEVICES = 64;
+static double* g_scratch_dev[MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on the current device (looked up on first use)
+static int scratch_setup(int* dev_out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(MWF_E_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MWF_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *dev_out = dev;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // single model handle
 // ---------------------------------------------------------------------------
 struct mwf_handle {
     mwf_params p{};
+    int dev = 0;               // the GPU the handle lives on
     hipStream_t st = nullptr;
     std::unique_ptr<SizeClass> cls;
     double* phys = nullptr;
@@ -520,8 +553,8 @@ extern "C" int mwf_create(const mwf_params* p, mwf_handle** out)
     RCCHK(check_params(p));
     mwf_handle* h = new mwf_handle();
     h->p = *p;
-    int rc = 0;
-    if (hipStreamCreate(&h->st) != hipSuccess) rc = fail(MWF_E_HIP, "hipStreamCreate failed");
+    int rc = scratch_setup(&h->dev);
+    if (!rc && hipStreamCreate(&h->st) != hipSuccess) rc = fail(MWF_E_HIP, "hipStreamCreate failed");
     if (!rc) {
         h->cls.reset(new SizeClass());
         rc = h->cls->init({*p}, h->st);
@@ -618,6 +651,18 @@ extern "C" int mwf_energy(mwf_handle* h, double* e)
     HIPCHK(hipMemcpyAsync(e, h->cls->edev, sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return 0;
+}
+
+extern "C" int mwf_energy_noalloc(mwf_handle* h, double* e_dev)
+{
+    if (!h || !e_dev) return fail(MWF_E_ARG, "null argument");
+    SizeClass& k = *h->cls;
+    if ((long long)k.nepart * k.nmem > NOALLOC_PARTS)
+        return fail(MWF_E_ARG, "the handle's energy partials exceed the no-allocation scratch (" +
+                                   std::to_string(NOALLOC_PARTS) + " doubles)");
+    double* part = g_scratch_dev[h->dev];
+    if (!part) return fail(MWF_E_STATE, "no-allocation scratch not set up on this device");
+    return k.energy(h->st, part, e_dev);
 }
 
 static int ensure_phys(mwf_handle* h)
