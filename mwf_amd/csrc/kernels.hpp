@@ -1318,6 +1318,30 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
 // ---------------------------------------------------------------------------
 template <int LOGMM> constexpr int xs_of() { return 3 * (XTILE >> LOGMM) * group_lanes<(1 << LOGMM)>(); }
 
+// FP32 k_xstep streams the next NL panel straight into an LDS staging area
+// (global_load_lds, no registers) while the current line is transformed: the
+// staging copy of one panel (3 XTILE values, 48 KB) fits beside the transform
+// buffer in FP32, not in FP64.
+template <typename R> constexpr bool xstep_async() { return sizeof(R) == 4; }
+template <typename R> constexpr size_t xstep_stage_bytes() { return xstep_async<R>() ? 3 * (size_t)XTILE * sizeof(cplx_t<R>) : 0; }
+
+// Wave-cooperative copy of a contiguous panel of nch 16-byte chunks from
+// global memory into LDS, asynchronously: lane l of a wave moves chunk c0 + l
+// to dst + 16 (c0 + l), c0 a multiple of 64 taken by the wave.  nch is a
+// multiple of 64.  The data is in LDS once the wave's vector memory count
+// has drained and a barrier has passed.
+template <int T, typename CT>
+__device__ __forceinline__ void panel_to_lds(const CT* __restrict__ src, CT* dst, int nch, int tid)
+{
+    const int wave = tid >> 6, lane = tid & 63;
+    const char* g = reinterpret_cast<const char*>(src);
+    char* l = reinterpret_cast<char*>(dst);
+    for (int c0 = wave * 64; c0 < nch; c0 += T)
+        __builtin_amdgcn_global_load_lds(g + (size_t)(c0 + lane) * 16,
+                                         (__attribute__((address_space(3))) void*)(l + (size_t)c0 * 16), 16, 0, 0);
+}
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 template <typename R, int LOGMM, bool INV_ONLY, bool FLUX>
 __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t<R>* __restrict__ NL, double2* __restrict__ c,
                                               cplx_t<R>* __restrict__ onl, cplx_t<R>* __restrict__ U,
@@ -1342,6 +1366,8 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     CT* twMl = reg + (XS / G) * MM;            // pass twiddles of the MM-point transforms
     static_assert(ptw_size<MM>() % 2 == 0, "twPl must stay 16-byte aligned");
     double2* twPl = reinterpret_cast<double2*>(twMl + ptw_size<MM>()); // exp(2 pi i j/P), j <= 2 MM
+    constexpr bool ASYNC = xstep_async<R>() && !INV_ONLY;
+    CT* stage = reinterpret_cast<CT*>(twPl + 2 * MM + 1);               // [3 XTILE] next NL panel
 
     const Geo g = bt.geo[blockIdx.z];
     NL += blockIdx.z * bt.nl_stride;
@@ -1372,24 +1398,53 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
 
     if constexpr (!INV_ONLY) {
         // ---------------- forward x transform + spec2mpt ----------------
-#pragma unroll 1
-        for (int li = 0; li < (FLUX ? 6 : 3); li++) {
+        // NL lines in order: comp u, v, w (flux form: B then A of each);
+        // y-mode 0 has no v
+        constexpr int NLI = FLUX ? 6 : 3;
+        auto line_on = [&](int li) { return !(k == 0 && (FLUX ? li >> 1 : li) == 1); };
+        auto panel_of = [&](int li) {
             const int comp = FLUX ? li >> 1 : li, part = FLUX ? li & 1 : 0;
-            if (k == 0 && comp == 1) continue;
             const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
+            return nl_panel(g, NL, n0, slot + part * g.KK);
+        };
+        const int nch = nrows * (int)sizeof(CT) / 16; // 16-byte chunks of one panel
+        if constexpr (ASYNC) panel_to_lds<XS>(panel_of(0), stage, nch, tid);
+#pragma unroll 1
+        for (int li = 0; li < NLI; li++) {
+            const int comp = FLUX ? li >> 1 : li, part = FLUX ? li & 1 : 0;
+            if (!line_on(li)) continue;
             // opaque copy of the thread index: keeps item arithmetic inside the loop
             int tidv = tid;
             asm volatile("" : "+v"(tidv));
-            const CT* pan = nl_panel(g, NL, n0, slot + part * g.KK);
+            if constexpr (ASYNC) {
+                // this line's panel has landed in the staging area: lay it out
+                // for the transforms, then start the next line's panel
+                wait_vmem();
+                __syncthreads();
 #pragma unroll
-            for (int q = 0; q < LPT; q++) {
-                const int i = tidv + XS * q;
-                if (i >= nrows) continue;
-                const int x = i >> g.lgNT, t = i & (NT - 1);
-                const int a = x / 3, b = x - 3 * a;
-                reg[(t * 3 + b) * MM + pidx(a)] = ld_stream(pan + i);
+                for (int q = 0; q < LPT; q++) {
+                    const int i = tidv + XS * q;
+                    if (i >= nrows) continue;
+                    const int x = i >> g.lgNT, t = i & (NT - 1);
+                    const int a = x / 3, b = x - 3 * a;
+                    reg[(t * 3 + b) * MM + pidx(a)] = stage[i];
+                }
+                __syncthreads();
+                int ln = li + 1;
+                while (ln < NLI && !line_on(ln)) ln++;
+                if (ln < NLI) panel_to_lds<XS>(panel_of(ln), stage, nch, tid);
+            } else {
+                const CT* pan = panel_of(li);
+#pragma unroll
+                for (int q = 0; q < LPT; q++) {
+                    const int i = tidv + XS * q;
+                    if (i >= nrows) continue;
+                    const int x = i >> g.lgNT, t = i & (NT - 1);
+                    const int a = x / 3, b = x - 3 * a;
+                    reg[(t * 3 + b) * MM + pidx(a)] = ld_stream(pan + i);
+                }
+                __syncthreads();
             }
-            __syncthreads();
             if (seq_on) {
                 CT v[PL];
 #pragma unroll

@@ -38,7 +38,7 @@ template <typename R, int LOGMM> static size_t xstep_lds()
 {
     constexpr size_t MM = (size_t)1 << LOGMM;
     return ((xs_of<LOGMM>() / group_lanes<(1 << LOGMM)>()) * MM + ptw_size<(1 << LOGMM)>()) * sizeof(cplx_t<R>) +
-           (2 * MM + 1) * sizeof(double2);
+           (2 * MM + 1) * sizeof(double2) + xstep_stage_bytes<R>();
 }
 
 template <typename R, int LOGMM, bool FLUX>
