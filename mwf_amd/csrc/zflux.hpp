@@ -16,10 +16,15 @@
 // Block: T = max(512, NN) threads, XB = T/NN x-rows; thread (xl, r) owns the
 // points z = 3r + b, b = 0..2, of row xl, and the output index n = r of that
 // row in every line, so a line whose rows straddle two batches is summed in
-// the thread's registers.  Sequences of ZP pairs per batch, transformed in
-// LDS either in place (ip_fft: one buffer, at or under 80 KB, so that two
-// blocks can share a CU) or ping-pong (pp_fft: two buffers, one barrier per
-// pass instead of two, for the form that runs one block per CU).
+// the thread's registers.  Sequences of ZP pairs per batch; lane group g of
+// G = min(64, NN/8) lanes transforms sequence g of the batch in place with
+// wave-local exchanges only (group_fft), so the radix passes need no block
+// barrier: a batch has one barrier between its fold and its transforms and
+// one between the transforms and the unfold (NN = 1024 and the FP64 one-
+// buffer form, whose registers do not hold a lane group's values, keep
+// block-wide Stockham passes).  With two buffers (PP) batches
+// alternate between them and need no third barrier; one buffer (at or under
+// 80 KB, so that two blocks can share a CU) adds a barrier after the unfold.
 #pragma once
 #include "kernels.hpp"
 #include "flux_gen.hpp"
@@ -59,17 +64,35 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     constexpr int FT = XB * ZP * H / T;               // fold tasks per thread
     static_assert(FT * T == XB * ZP * H, "fold tasks must tile the block");
     extern __shared__ unsigned char smem_raw[];
-    CT* buf = reinterpret_cast<CT*>(smem_raw);        // [SEQ][NN]: input of each transform
-    CT* bufB = buf + (PP ? SEQ * NN : 0);             // [SEQ][NN]: the ping-pong partner
+    CT* bufA = reinterpret_cast<CT*>(smem_raw);       // [SEQ][NN]: the sequences of a batch
+    CT* bufB = bufA + (PP ? SEQ * NN : 0);            // [SEQ][NN]: those of the next batch (PP)
     CT* twNl = bufB + SEQ * NN;                       // [ptw_size<NN>()] pass twiddles
-    // where a transform's result lies: in place, or after an odd number of
-    // ping-pong passes in the partner buffer
-    constexpr bool res_in_buf = !(PP && fft_in_b<LOGNN>());
-    CT* res = res_in_buf ? buf : bufB;
-    auto transform = [&](auto sign, int S, int t) {
-        if constexpr (PP) pp_fft<CT, LOGNN, decltype(sign)::value, T, true>(buf, bufB, S, twNl, t);
-        else ip_fft<CT, LOGNN, decltype(sign)::value, T, SEQ, true>(buf, S, twNl, t);
+    // buffer of batch k (inverse batches first, then forward ones)
+    auto buf_of = [&](int k) { return (PP && (k & 1)) ? bufB : bufA; };
+    // transforms of the first S sequences of b, in place, natural order in
+    // and out; ends with a block barrier.  Lane-group transforms where their
+    // registers fit (NN <= 512, and FP32 or the two-wave form); otherwise
+    // Stockham passes over the whole block (ip_fft, a barrier per half-pass).
+    constexpr bool WAVE = LOGNN <= 9 && (PP || sizeof(R) == 4);
+    auto transform = [&](auto sign, int S, CT* b, int t) {
+        if constexpr (!WAVE) {
+            ip_fft<CT, LOGNN, decltype(sign)::value, T, SEQ, true>(b, S, twNl, t);
+            return;
+        }
+        constexpr int G = group_lanes<NN>(), PL = NN / G;
+        const int gi = t / G, gl = t % G;
+        if (gi < S) {
+            CT* sq = b + gi * NN;
+            CT v[PL];
+#pragma unroll
+            for (int q = 0; q < PL; q++) v[q] = sq[pidx(gl + G * q)];
+            group_fft<CT, LOGNN, decltype(sign)::value, true>(v, sq, gl, twNl);
+#pragma unroll
+            for (int q = 0; q < PL; q++) sq[pidx(gl + G * q)] = v[q];
+        }
+        __syncthreads();
     };
+    static_assert(SEQ * group_lanes<NN>() <= T, "one lane group per sequence");
     CT* twQl = twNl + ptw_size<NN>();                 // [NN]: w^k, k < NN
 
     const int tid = threadIdx.x;
@@ -127,6 +150,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         constexpr int NV = NPI - BN * ZP < ZP ? NPI - BN * ZP : ZP; // valid pairs
         const int tid = opaque_tid();
         const int xl = tid >> LOGNN, r = tid & (NN - 1);
+        CT* buf = buf_of(BN);
 #pragma unroll
         for (int f = 0; f < FT; f++) {
             const int tau = tid + T * f;
@@ -142,20 +166,19 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         }
         if constexpr (BN + 1 < NBI) prefetch(BN + 1);
         __syncthreads();
-        transform(std::integral_constant<int, +1>{}, nseq(NV), tid);
+        transform(std::integral_constant<int, +1>{}, nseq(NV), buf, tid);
         static_for<0, NV>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
             constexpr int s1 = 2 * (BN * ZP + p);
 #pragma unroll
             for (int b = 0; b < 3; b++) {
-                const CT zv = res[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
+                const CT zv = buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
                 vb[b][s1] = zv.x;
                 if constexpr (s1 + 1 < KK) vb[b][s1 + 1] = zv.y;
             }
         });
-        // the next batch writes buf: a barrier is needed only if the result
-        // was read from buf itself
-        if constexpr (res_in_buf) __syncthreads();
+        // with one buffer the next batch overwrites this one's results
+        if constexpr (!PP) __syncthreads();
     });
 
     // ---- products, forward z transforms (r2c, -1), line assembly ----
@@ -168,6 +191,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         constexpr int NV = NPO - RD * ZP < ZP ? NPO - RD * ZP : ZP; // valid output pairs
         const int tid = opaque_tid();
         const int xl = tid >> LOGNN, r = tid & (NN - 1);
+        CT* buf = buf_of(NBI + RD);
         // the base values pass through an empty asm per batch: products shared
         // by rows of different batches are formed again in each batch instead
         // of being computed once and held in registers across all of them
@@ -188,7 +212,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             }
         });
         __syncthreads();
-        transform(std::integral_constant<int, -1>{}, nseq(NV), tid);
+        transform(std::integral_constant<int, -1>{}, nseq(NV), buf, tid);
         static_for<0, NV>([&](auto pi) {
             constexpr int p = decltype(pi)::value;
             constexpr int q1 = 2 * (RD * ZP + p);
@@ -196,8 +220,8 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             CT X[2] = {CT{(R)0, (R)0}, CT{(R)0, (R)0}};
 #pragma unroll
             for (int b = 0; b < 3; b++) {
-                const CT zn = res[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
-                const CT zm = res[((xl * ZP + p) * 3 + b) * NN + pidx(rm)];
+                const CT zn = buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
+                const CT zm = buf[((xl * ZP + p) * 3 + b) * NN + pidx(rm)];
                 const CT F1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
                 const CT F2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
                 if (b == 0) {
@@ -230,6 +254,6 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
                 }
             });
         });
-        if constexpr (res_in_buf) __syncthreads();
+        if constexpr (!PP) __syncthreads();
     });
 }
