@@ -279,8 +279,11 @@ __device__ __forceinline__ void wave_lds_sync()
 
 template <int L> constexpr int group_lanes() { return L / 8 < 64 ? L / 8 : 64; }
 
+// xo: a constant XORed into every slot of the group's region (a
+// per-sequence swizzle of the caller's layout; any xo < 16 keeps the access
+// patterns of the passes free of bank conflicts).
 template <typename CT, int LOG, int SIGN, bool PTW = false, int Ns = 1, int PASS = 0>
-__device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, const CT* __restrict__ tw)
+__device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, const CT* __restrict__ tw, int xo = 0)
 {
     constexpr int L = 1 << LOG;
     constexpr int G = group_lanes<L>();
@@ -293,7 +296,7 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
         for (int i = 0; i < NBL; i++) {
             const int j = gl + G * i;
 #pragma unroll
-            for (int r = 0; r < R; r++) x[i][r] = PASS == 0 ? v[i + r * NBL] : reg[pidx(j + r * (L / R))];
+            for (int r = 0; r < R; r++) x[i][r] = PASS == 0 ? v[i + r * NBL] : reg[pidx(j + r * (L / R)) ^ xo];
         }
         if constexpr (PASS > 0) wave_lds_sync();
 #pragma unroll
@@ -315,10 +318,10 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
                 const int k = j & (Ns - 1);
                 const int o = (j - k) * R + k;
 #pragma unroll
-                for (int r = 0; r < R; r++) reg[pidx(o + r * Ns)] = x[i][r];
+                for (int r = 0; r < R; r++) reg[pidx(o + r * Ns) ^ xo] = x[i][r];
             }
             wave_lds_sync();
-            group_fft<CT, LOG, SIGN, PTW, Ns * R, PASS + 1>(v, reg, gl, tw);
+            group_fft<CT, LOG, SIGN, PTW, Ns * R, PASS + 1>(v, reg, gl, tw, xo);
         }
     }
 }

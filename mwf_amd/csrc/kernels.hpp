@@ -1323,6 +1323,21 @@ template <int LOGMM> constexpr int xs_of() { return 3 * (XTILE >> LOGMM) * group
 // staging copy of one panel (3 XTILE values, 48 KB) fits beside the transform
 // buffer in FP32, not in FP64.
 template <typename R> constexpr bool xstep_async() { return sizeof(R) == 4; }
+
+// Sequence (t, b) of the x kernels' transform buffer holds element a at slot
+// pidx(a) ^ xsw(t, b).  The decimating transposes between panels ([x][t],
+// x = 3a + b) and sequences have up to 16 lanes of one bank phase on one a;
+// in FP32 this per-sequence XOR sends them to distinct banks (conflict-free
+// at MM <= 256, at most 2-way at 512 and 1024, from 6- to 16-way
+// unswizzled), while accesses within one sequence keep their conflict-free
+// patterns.  FP64 keeps the plain layout: there the XOR ((t * 5) ^ b) & 7
+// would be as effective, but its index registers push k_xstep from 17 to 25
+// spilled VGPRs at MM = 512, which costs more (4 %) than the conflicts.
+template <typename R> __device__ __forceinline__ int xsw(int t, int b)
+{
+    if constexpr (sizeof(R) == 4) return ((t * 9) ^ (b * 3)) & 15; // 8-byte slots, 16 per bank phase
+    else return 0;
+}
 template <typename R> constexpr size_t xstep_stage_bytes() { return xstep_async<R>() ? 3 * (size_t)XTILE * sizeof(cplx_t<R>) : 0; }
 
 // Wave-cooperative copy of a contiguous panel of nch 16-byte chunks from
@@ -1384,6 +1399,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     const int s = tid / G, gl = tid % G;
     const bool seq_on = s < 3 * NT;
     CT* myreg = reg + s * MM;
+    const int sxo = xsw<R>(s / 3, s % 3); // swizzle of this thread's group sequence
     const double ak1 = ak1_of(k, g.beta);
     const double ak2 = ak2_of(k, g.beta);
 
@@ -1427,7 +1443,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                     if (i >= nrows) continue;
                     const int x = i >> g.lgNT, t = i & (NT - 1);
                     const int a = x / 3, b = x - 3 * a;
-                    reg[(t * 3 + b) * MM + pidx(a)] = stage[i];
+                    reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = stage[i];
                 }
                 __syncthreads();
                 int ln = li + 1;
@@ -1441,17 +1457,17 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                     if (i >= nrows) continue;
                     const int x = i >> g.lgNT, t = i & (NT - 1);
                     const int a = x / 3, b = x - 3 * a;
-                    reg[(t * 3 + b) * MM + pidx(a)] = ld_stream(pan + i);
+                    reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = ld_stream(pan + i);
                 }
                 __syncthreads();
             }
             if (seq_on) {
                 CT v[PL];
 #pragma unroll
-                for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
-                group_fft<CT, LOGMM, -1, true>(v, myreg, gl, twMl);
+                for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q) ^ sxo];
+                group_fft<CT, LOGMM, -1, true>(v, myreg, gl, twMl, sxo);
 #pragma unroll
-                for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
+                for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q) ^ sxo] = v[q];
             }
             __syncthreads();
 #pragma unroll
@@ -1461,11 +1477,11 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                 const int t = it >> LOGMM, r = it & (MM - 1);
                 const int n = n0 + t;
                 const CT* bt3 = reg + t * 3 * MM;
-                const CT F0 = bt3[pidx(r)];
+                const CT F0 = bt3[pidx(r) ^ xsw<R>(t, 0)];
                 double2 Xp = double2{(double)F0.x, (double)F0.y}, Xn = Xp;
 #pragma unroll
                 for (int b = 1; b < 3; b++) {
-                    const CT Fb = bt3[b * MM + pidx(r)];
+                    const CT Fb = bt3[b * MM + (pidx(r) ^ xsw<R>(t, b))];
                     const double fr = (double)Fb.x, fi = (double)Fb.y;
                     const double2 w1 = twPl[r * b];        // m = r:      times conj(w1)
                     const double2 w2 = twPl[(MM - r) * b]; // m = r - MM: times w2
@@ -1588,23 +1604,23 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                 X[h] = val;
             }
             CT* bt3 = reg + t * 3 * MM;
-            bt3[pidx(r)] = CT{(R)(X[0].x + X[1].x), (R)(X[0].y + X[1].y)};
+            bt3[pidx(r) ^ xsw<R>(t, 0)] = CT{(R)(X[0].x + X[1].x), (R)(X[0].y + X[1].y)};
 #pragma unroll
             for (int b = 1; b < 3; b++) {
                 const double2 w1 = twPl[r * b];
                 const double2 w2 = twPl[(MM - r) * b]; // conj: exp(2 pi i (r-MM) b / P)
                 const double2 y = xfold_b(X[0], X[1], w1, w2);
-                bt3[b * MM + pidx(r)] = CT{(R)y.x, (R)y.y};
+                bt3[b * MM + (pidx(r) ^ xsw<R>(t, b))] = CT{(R)y.x, (R)y.y};
             }
         }
         __syncthreads();
         if (seq_on) {
             CT v[PL];
 #pragma unroll
-            for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q)];
-            group_fft<CT, LOGMM, +1, true>(v, myreg, gl, twMl);
+            for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q) ^ sxo];
+            group_fft<CT, LOGMM, +1, true>(v, myreg, gl, twMl, sxo);
 #pragma unroll
-            for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q)] = v[q];
+            for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q) ^ sxo] = v[q];
         }
         __syncthreads();
         CT* Ul = U + (size_t)line * g.NN + n0;
@@ -1615,7 +1631,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             if (i >= nrows) continue;
             const int x = i >> g.lgNT, t = i & (NT - 1);
             const int a = x / 3, b = x - 3 * a;
-            st_piece(Ul + x * ustride + t, reg[(t * 3 + b) * MM + pidx(a)], wide);
+            st_piece(Ul + x * ustride + t, reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))], wide);
         }
         __syncthreads();
     }
