@@ -24,7 +24,21 @@
 #include "common.hpp"
 
 __host__ __device__ constexpr int lpad(int L) { return L; }
-__device__ __forceinline__ int pidx(int i) { return i ^ ((i >> 3) & 15); }
+__host__ __device__ constexpr int pidx(int i) { return i ^ ((i >> 3) & 15); }
+
+// The swizzle is XOR-linear over disjoint bit fields: for a lane index a and a
+// constant c with (a & c) == 0, pidx(a + c) = pidx(a) ^ pidx(c), and since
+// pidx changes only bits 0..3, pidx(a) ^ pidx(c) = (pidx(a) ^ lo(c)) + hi(c)
+// with lo(c) = pidx(c) & 15 and hi(c) = c & ~15.  Every butterfly address is
+// such a sum (j < L/R below r L/R; o = (j - k) R + k around r Ns), so a pass
+// computes pidx of its lane part once, and each of its R accesses costs at
+// most one XOR with a small constant, the rest being the instruction's
+// immediate offset.  swz<c>(base, pa) is base[pidx(a + c)] given pa = pidx(a).
+template <int C, typename CT> __device__ __forceinline__ CT& swz(CT* base, int pa)
+{
+    constexpr int LO = pidx(C) & 15, HI = C & ~15;
+    return (base + (pa ^ LO))[HI];
+}
 
 template <int SIGN, typename CT> __device__ __forceinline__ void dft2(CT* x)
 {
@@ -189,8 +203,8 @@ template <typename CT, int L, int R, int Ns, int SIGN, bool PTW = false>
 __device__ __forceinline__ void bfly_load(const CT* __restrict__ sb, int j, CT* x, const CT* __restrict__ tw)
 {
     constexpr int NB = L / R;
-#pragma unroll
-    for (int r = 0; r < R; r++) x[r] = sb[pidx(j + r * NB)];
+    const int pj = pidx(j);
+    static_for<0, R>([&](auto ri) { x[ri] = swz<decltype(ri)::value * NB>(sb, pj); });
     if constexpr (Ns > 1) twiddle_j<L, R, Ns, SIGN, PTW>(x, tw, j);
     dftR<R, SIGN>(x);
 }
@@ -199,9 +213,8 @@ template <typename CT, int R, int Ns>
 __device__ __forceinline__ void bfly_store(CT* __restrict__ db, int j, const CT* x)
 {
     const int k = j & (Ns - 1);
-    const int o = (j - k) * R + k;
-#pragma unroll
-    for (int r = 0; r < R; r++) db[pidx(o + r * Ns)] = x[r];
+    const int po = pidx((j - k) * R + k);
+    static_for<0, R>([&](auto ri) { swz<decltype(ri)::value * Ns>(db, po) = x[ri]; });
 }
 
 template <int L, int Ns = 1> constexpr int npasses()
@@ -295,8 +308,13 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
 #pragma unroll
         for (int i = 0; i < NBL; i++) {
             const int j = gl + G * i;
+            if constexpr (PASS == 0) {
 #pragma unroll
-            for (int r = 0; r < R; r++) x[i][r] = PASS == 0 ? v[i + r * NBL] : reg[pidx(j + r * (L / R)) ^ xo];
+                for (int r = 0; r < R; r++) x[i][r] = v[i + r * NBL];
+            } else {
+                const int pj = pidx(j) ^ xo;
+                static_for<0, R>([&](auto ri) { x[i][ri] = swz<decltype(ri)::value * (L / R)>(reg, pj); });
+            }
         }
         if constexpr (PASS > 0) wave_lds_sync();
 #pragma unroll
@@ -316,9 +334,8 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
             for (int i = 0; i < NBL; i++) {
                 const int j = gl + G * i;
                 const int k = j & (Ns - 1);
-                const int o = (j - k) * R + k;
-#pragma unroll
-                for (int r = 0; r < R; r++) reg[pidx(o + r * Ns) ^ xo] = x[i][r];
+                const int po = pidx((j - k) * R + k) ^ xo;
+                static_for<0, R>([&](auto ri) { swz<decltype(ri)::value * Ns>(reg, po) = x[i][ri]; });
             }
             wave_lds_sync();
             group_fft<CT, LOG, SIGN, PTW, Ns * R, PASS + 1>(v, reg, gl, tw, xo);

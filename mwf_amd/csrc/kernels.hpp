@@ -1411,6 +1411,9 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
 
     ptw_fill<CT, LOGMM>(twMl, twMM, tid, XS);
     for (int i = tid; i <= 2 * MM; i += XS) twPl[i] = twP[i];
+    // the tables are read by other threads; the inverse-only form reads twPl
+    // before its first transform barrier
+    __syncthreads();
 
     if constexpr (!INV_ONLY) {
         // ---------------- forward x transform + spec2mpt ----------------
@@ -1463,11 +1466,10 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             }
             if (seq_on) {
                 CT v[PL];
-#pragma unroll
-                for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q) ^ sxo];
+                const int pg = pidx(gl) ^ sxo;
+                static_for<0, PL>([&](auto qi) { v[qi] = swz<decltype(qi)::value * G>(myreg, pg); });
                 group_fft<CT, LOGMM, -1, true>(v, myreg, gl, twMl, sxo);
-#pragma unroll
-                for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q) ^ sxo] = v[q];
+                static_for<0, PL>([&](auto qi) { swz<decltype(qi)::value * G>(myreg, pg) = v[qi]; });
             }
             __syncthreads();
 #pragma unroll
@@ -1616,11 +1618,10 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
         __syncthreads();
         if (seq_on) {
             CT v[PL];
-#pragma unroll
-            for (int q = 0; q < PL; q++) v[q] = myreg[pidx(gl + G * q) ^ sxo];
+            const int pg = pidx(gl) ^ sxo;
+            static_for<0, PL>([&](auto qi) { v[qi] = swz<decltype(qi)::value * G>(myreg, pg); });
             group_fft<CT, LOGMM, +1, true>(v, myreg, gl, twMl, sxo);
-#pragma unroll
-            for (int q = 0; q < PL; q++) myreg[pidx(gl + G * q) ^ sxo] = v[q];
+            static_for<0, PL>([&](auto qi) { swz<decltype(qi)::value * G>(myreg, pg) = v[qi]; });
         }
         __syncthreads();
         CT* Ul = U + (size_t)line * g.NN + n0;

@@ -84,11 +84,10 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         if (gi < S) {
             CT* sq = b + gi * NN;
             CT v[PL];
-#pragma unroll
-            for (int q = 0; q < PL; q++) v[q] = sq[pidx(gl + G * q)];
+            const int pg = pidx(gl);
+            static_for<0, PL>([&](auto qi) { v[qi] = swz<decltype(qi)::value * G>(sq, pg); });
             group_fft<CT, LOGNN, decltype(sign)::value, true>(v, sq, gl, twNl);
-#pragma unroll
-            for (int q = 0; q < PL; q++) sq[pidx(gl + G * q)] = v[q];
+            static_for<0, PL>([&](auto qi) { swz<decltype(qi)::value * G>(sq, pg) = v[qi]; });
         }
         __syncthreads();
     };
