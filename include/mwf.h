@@ -163,7 +163,7 @@ int mwf_blend(mwf_handle *a, mwf_handle *b, double d, mwf_handle *dst);
  * handle or ensemble class of each (precision, k0, mm, nn) times its
  * candidate kernel plans at creation (a few steps each, on its zero state)
  * and keeps the fastest, and later ones of that size reuse the choice, so
- * results stay reproducible within a process; 2..33 = force plan choice
+ * results stay reproducible within a process; 2..65 = force plan choice
  * mode - 2 where that alternative exists (for tests and plan comparisons). */
 int mwf_autotune(int32_t mode);
 /* The plan a handle runs: bit 0 = phase-sequential z kernel instead of the
@@ -173,7 +173,10 @@ int mwf_autotune(int32_t mode);
  * (conservative) form of vel_nonlinear's products: the same projected
  * (u.grad)u, with d/dx, d/dz taken in spectral space and d/dy moved onto the
  * wall-normal bases, equal to the advective form up to rounding; bit 4 (with
- * bit 3) = its z kernel compiled for two waves per SIMD instead of four. */
+ * bit 3) = its z kernel compiled for two waves per SIMD instead of four;
+ * bit 5 (advective form, NN = 1024; implies bit 1) = the phase-sequential z
+ * kernel with the three fold phases of an x-row in three blocks, each writing
+ * its own NL partial, which the x-forward kernel sums. */
 int mwf_plan_info(mwf_handle *h, int32_t *choice);
 
 /* Wait for all queued work of the handle. */

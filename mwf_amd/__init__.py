@@ -126,11 +126,11 @@ def _p(a):
 def autotune(mode=1):
     """Process-wide plan selection for handles created afterwards (mwf_autotune),
     the role of tra_precompute's FFTW_PATIENT planning: 0 = rule-based plans,
-    1 = timed choice per size, 2..33 = force plan bits mode - 2."""
+    1 = timed choice per size, 2..65 = force plan bits mode - 2."""
     _check(lib().mwf_autotune(int(mode)))
 
 
-PLAN_ZB, PLAN_SPLIT_X, PLAN_Z2, PLAN_FLUX, PLAN_FLUX_W2 = 1, 2, 4, 8, 16
+PLAN_ZB, PLAN_SPLIT_X, PLAN_Z2, PLAN_FLUX, PLAN_FLUX_W2, PLAN_ZB_SPLIT = 1, 2, 4, 8, 16, 32
 
 
 def dims(p):
@@ -281,7 +281,8 @@ class Velocity:
 
     def plan_info(self):
         """Plan bits: PLAN_ZB (phase-sequential z kernel), PLAN_SPLIT_X (separate x kernels),
-        PLAN_Z2 (two-points-per-thread z kernel)."""
+        PLAN_Z2 (two-points-per-thread z kernel), PLAN_FLUX (flux form), PLAN_FLUX_W2 (its
+        two-wave z kernel), PLAN_ZB_SPLIT (the fold phases of a row in three blocks)."""
         ch = ctypes.c_int32()
         _check(lib().mwf_plan_info(self._h, ctypes.byref(ch)))
         return ch.value

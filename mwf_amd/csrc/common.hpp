@@ -85,6 +85,7 @@ struct Batch {
     size_t u_stride;     // complex elements between members' U
     size_t nl_stride;    // complex elements between members' NL
     int ep_stride;       // energy partials per member
+    size_t nl_part = 0;  // complex elements between the three fold-phase partials of NL (0: NL is whole)
 };
 
 // Workgroups of a launch go round-robin to the 8 XCDs (block b on XCD b % 8),

@@ -836,6 +836,14 @@ __global__ __launch_bounds__(Z2T) void k_z2(Batch bt, const cplx_t<R>* __restric
 // NL therefore holds partial sums between phases; each element is read and
 // written only by the thread that owns it, in phase order, which makes the
 // result identical to the three-phase sum of k_z.
+//
+// PH3: the three phases of an x-row run in three blocks instead, each writing
+// its unscaled share to its own NL partial (bt.nl_part apart); the x kernel
+// sums ((P0 + P1) + P2) and scales by 1/(PQ) as it loads NL.  Block B maps to
+// XCD B % 8 and, there, to phase (B / 8) % 3 of row (B / 24): the three
+// phases of a row are dispatched back to back on one XCD and read the row's U
+// lines through its L2 once, with no read-modify-write of NL; each XCD takes a
+// contiguous run of rows, so rows sharing NL lines meet in one L2.
 // ---------------------------------------------------------------------------
 constexpr int ZB = 1024;
 template <typename R> constexpr int zb_seq() { return 8; } // sequences per batch
@@ -851,7 +859,7 @@ template <typename R, int LOGNN> constexpr size_t zb_lds_bytes()
     return sizeof(cplx_t<R>) * ((size_t)zb_seq<R>() * ZB + ptw_size<(1 << LOGNN)>() + (zb_twl<R>() ? 2 * ((size_t)1 << LOGNN) : 0));
 }
 
-template <typename R, int K0, int LOGNN>
+template <typename R, int K0, int LOGNN, bool PH3 = false>
 __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict__ U, cplx_t<R>* __restrict__ NL,
                                            const cplx_t<R>* __restrict__ twNN, const cplx_t<R>* __restrict__ twQ)
 {
@@ -886,11 +894,21 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
     const CT* tN = twNl;
     const CT* tQ = zb_twl<R>() ? twQl : twQ;
 
-    const int x0 = xcd_tile(blockIdx.x, gridDim.x) * XB;
+    int xt = 0, b_first = 0, b_end = 3;
+    if constexpr (PH3) {
+        const int B = blockIdx.x, idx = B >> 3, nb = gridDim.x / 3;
+        b_first = idx % 3;
+        b_end = b_first + 1;
+        xt = (B & 7) * (nb >> 3) + idx / 3;
+        NL += b_first * bt.nl_part;
+    } else {
+        xt = xcd_tile(blockIdx.x, gridDim.x);
+    }
+    const int x0 = xt * XB;
     const R scale = (R)g.scale;
 
 #pragma unroll 1
-    for (int b = 0; b < 3; b++) {
+    for (int b = b_first; b < b_end; b++) {
         R vb[KK], an[KK];
 #pragma unroll
         for (int s = 0; s < KK; s++) an[s] = (R)0;
@@ -955,7 +973,7 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
             // FP32: the partial sums of the earlier phases are loaded before the
             // transform, so that their latency hides behind it (FP64 has no
             // registers to spare for them and loads them after)
-            constexpr bool PRE = sizeof(R) == 4;
+            constexpr bool PRE = sizeof(R) == 4 && !PH3;
             CT o1[ZO_], o2[ZO_];
             if (PRE && b > 0 && valid) {
                 static_for<0, ZO_>([&](auto pi) {
@@ -989,7 +1007,10 @@ __global__ __launch_bounds__(ZB) void k_zb(Batch bt, const cplx_t<R>* __restrict
                         X1 = cmul(X1, w);
                         X2 = cmul(X2, w);
                     }
-                    if (valid) {
+                    if (valid && PH3) {
+                        NL[nidx(g, x, c1, r)] = X1;
+                        if constexpr (c2 < KK) NL[nidx(g, x, c2, r)] = X2;
+                    } else if (valid) {
                         if (b > 0) X1 = cadd(PRE ? o1[p] : NL[nidx(g, x, c1, r)], X1);
                         NL[nidx(g, x, c1, r)] = b == 2 ? cscale(X1, scale) : X1;
                         if constexpr (c2 < KK) {
@@ -1206,7 +1227,12 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
             if (it >= 3 * nitems) continue;
             const int x = it >> g.lgNT, t = it & (NT - 1);
             const int a = x / 3, b = x - 3 * a;
-            buf[((size_t)t * 3 + b) * LP + pidx(a)] = pan[it];
+            CT v = pan[it];
+            if (bt.nl_part) { // the z pass's three fold-phase partials: ((P0 + P1) + P2) / (PQ)
+                const CT v1 = pan[it + bt.nl_part], v2 = pan[it + 2 * bt.nl_part];
+                v = cscale(cadd(cadd(v, v1), v2), (R)g.scale);
+            }
+            buf[((size_t)t * 3 + b) * LP + pidx(a)] = v;
         }
         __syncthreads();
         ip_fft<CT, LOGMM, -1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);

@@ -158,7 +158,9 @@ template <typename T> static int dev_upload(T** d, const T* h, size_t n)
 // thread z kernel (k_z2; takes precedence over bit 1), 8 = flux form of the
 // nonlinear term (k_zc and the flux x kernels; takes precedence over bits 1
 // and 4, where it exists), 16 = with bit 8, the flux z kernel compiled for
-// 2 waves per SIMD (no spills) instead of 4 (two blocks per CU).
+// 2 waves per SIMD (no spills) instead of 4 (two blocks per CU), 32 = at
+// NN = 1024 in the advective form, k_zb with the fold phases of a row in
+// three blocks and three NL partials (with the split x kernels).
 // ---------------------------------------------------------------------------
 static std::atomic<int> g_tune{0};
 static std::mutex g_tune_mu;
@@ -217,7 +219,8 @@ struct SizeClass {
         HIPCHK(hipMalloc(&onl, hsz * mpt_elems * nmem));
         HIPCHK(hipMalloc((void**)&cs, sizeof(double2) * mpt_elems));
         HIPCHK(hipMalloc(&U, rsz * u_el * nmem));
-        HIPCHK(hipMalloc(&NL, rsz * nl_el * nmem));
+        // three NL partials where the phase-split z kernel may run
+        HIPCHK(hipMalloc(&NL, rsz * nl_el * nmem * (base.zf_ph3 ? 3 : 1)));
         HIPCHK(hipMalloc((void**)&epart, sizeof(double) * nepart * nmem));
         HIPCHK(hipMalloc((void**)&edev, sizeof(double) * nmem));
         bt = Batch{geos, mpt_elems, u_el, nl_el, nepart};
@@ -256,7 +259,11 @@ struct SizeClass {
         } else if ((ch & 1) && base.zf_alt) {
             plan.zf = base.zf_alt;
         }
-        if (ch & 2) plan.xstep = nullptr;
+        // phase-split z kernel: three NL partials, summed by the split x kernels
+        const bool ph3 = !flux && (ch & 32) && base.zf_ph3;
+        if (ph3) plan.zf = base.zf_ph3;
+        if ((ch & 2) || ph3) plan.xstep = nullptr;
+        bt.nl_part = ph3 ? bt.nl_stride * nmem : 0;
         const int UL = flux ? g0.KK : 2 * g0.KK, NLL = flux ? 2 * g0.KK : g0.KK;
         if (g0.UL != UL) {
             g0.UL = UL;
@@ -295,6 +302,7 @@ struct SizeClass {
         if (base.zf_alt2) zc.push_back(4);
         if (base.zc) zc.push_back(8);
         if (base.zc_alt) zc.push_back(8 | 16);
+        if (base.zf_ph3) cand.push_back(32 | 2);
         for (int z : zc) {
             if ((z & 8) ? (bool)base.xstep_f : (bool)base.xstep) cand.push_back(z);
             cand.push_back(z | 2);
@@ -911,7 +919,7 @@ extern "C" int mwf_blend(mwf_handle* a, mwf_handle* b, double d, mwf_handle* dst
 
 extern "C" int mwf_autotune(int32_t mode)
 {
-    if (mode < 0 || mode > 33) return fail(MWF_E_ARG, "mode must be 0 (rules), 1 (timed) or 2..33 (forced plan)");
+    if (mode < 0 || mode > 65) return fail(MWF_E_ARG, "mode must be 0 (rules), 1 (timed) or 2..65 (forced plan)");
     g_tune.store(mode);
     return 0;
 }
@@ -922,7 +930,8 @@ extern "C" int mwf_plan_info(mwf_handle* h, int32_t* choice)
     const SizeClass& k = *h->cls;
     *choice = ((k.base.zf_alt && k.plan.zf == k.base.zf_alt) ? 1 : 0) | (k.plan.xstep ? 0 : 2) |
               ((k.base.zf_alt2 && k.plan.zf == k.base.zf_alt2) ? 4 : 0) | (k.flux() ? 8 : 0) |
-              ((k.base.zc_alt && k.plan.zf == k.base.zc_alt) ? 16 : 0);
+              ((k.base.zc_alt && k.plan.zf == k.base.zc_alt) ? 16 : 0) |
+              ((k.base.zf_ph3 && k.plan.zf == k.base.zf_ph3) ? 32 : 0);
     return 0;
 }
 

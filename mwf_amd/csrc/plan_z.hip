@@ -38,15 +38,18 @@ static hipError_t launch_z2(const Batch& bt, int nmem, const Geo& g, hipStream_t
     return hipGetLastError();
 }
 
-template <typename R, int K0, int LOGNN>
+template <typename R, int K0, int LOGNN, bool PH3 = false>
 static hipError_t launch_zb(const Batch& bt, int nmem, const Geo& g, hipStream_t s, const void* U, void* NL,
                             const void* twNN, const void* twQ)
 {
     using CT = cplx_t<R>;
     constexpr int XB = ZB >> LOGNN;
-    dim3 grid((g.P + XB - 1) / XB, 1, nmem);
-    hipLaunchKernelGGL((k_zb<R, K0, LOGNN>), grid, dim3(ZB), (zb_lds_bytes<R, LOGNN>()), s, bt, (const CT*)U, (CT*)NL,
-                       (const CT*)twNN, (const CT*)twQ);
+    const int nb = (g.P + XB - 1) / XB;
+    // PH3: three blocks per row block; the XCD mapping needs whole rounds of 8
+    if (PH3 && (nb % 8 != 0 || g.P % XB != 0 || bt.nl_part == 0)) return hipErrorInvalidValue;
+    dim3 grid(PH3 ? 3 * nb : nb, 1, nmem);
+    hipLaunchKernelGGL((k_zb<R, K0, LOGNN, PH3>), grid, dim3(ZB), (zb_lds_bytes<R, LOGNN>()), s, bt, (const CT*)U,
+                       (CT*)NL, (const CT*)twNN, (const CT*)twQ);
     return hipGetLastError();
 }
 
@@ -113,6 +116,7 @@ template <typename R, int K0> void pick_z(int ln, Plan& p)
     case 9: p.zf_alt = launch_zb<R, K0, 9>; break;
     case 10: // one x-row per 1024-thread block: the three fold phases in sequence
         p.zf = launch_zb<R, K0, 10>;
+        p.zf_ph3 = launch_zb<R, K0, 10, true>;
         p.zphys = launch_zphys<R, K0, 10>;
         p.zfwdp = launch_zfwdp<R, 10>;
         // the flux z kernel holds all three phases of a 3072-point row at
@@ -144,6 +148,7 @@ template <typename R, int K0> void prepare_z(int ln)
     case 9: set_lds(k_zb<R, K0, 9>, zb_lds_bytes<R, 9>()); break;
     case 10:
         set_lds(k_zb<R, K0, 10>, zb_lds_bytes<R, 10>());
+        set_lds(k_zb<R, K0, 10, true>, zb_lds_bytes<R, 10>());
         set_lds(k_zphys<R, K0, 10>, zphys_lds<R, 10>());
         set_lds(k_zfwd_phys<R, 10>, zfp_lds<R, 10>());
         if constexpr (std::is_same<R, float>::value)

@@ -28,6 +28,10 @@ struct Plan {
                          const void*) = nullptr;
     hipError_t (*zf_alt2)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*,
                           const void*) = nullptr;
+    // k_zb with the fold phases of a row in three blocks and three NL partials
+    // (bt.nl_part), summed by the split x kernels
+    hipError_t (*zf_ph3)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*,
+                         const void*) = nullptr;
     // flux form (zflux.hpp): z kernel and the x kernels that read its B/A lines
     // and write U without x-derivative lines
     hipError_t (*zc)(const Batch&, int, const Geo&, hipStream_t, const void*, void*, const void*, const void*) = nullptr;

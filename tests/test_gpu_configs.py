@@ -145,3 +145,32 @@ def test_mixed_energy_within_1e5(name, steps):
     st = rel_l2(v32.get_state(), v64.get_state())
     print(f"mixed {name}: {steps} steps, worst energy rel {worst:.3e}, final state rel_l2 {st:.3e}")
     assert worst <= 1e-5
+
+
+def test_m6_2048_phase_split_matches_phase_sequential():
+    """M6 2048^2 (the k_zb size), FP64: the phase-split z kernel (plan bits
+    32 | 2) against the phase-sequential one (pinned to the oracle at this size
+    by test_m6_2048_two_steps): nonlinear term and 5 steps, to rounding."""
+    p = cases.params("m6_2048")
+    seed, scl, k0ic = cases.ic_of("m6_2048")
+    re, im = mw.ic_gaussian(p, seed, scl, k0ic)
+    out = {}
+    try:
+        for plan in (2, 34):
+            mw.autotune(2 + plan)
+            v = mw.Velocity(p)
+            assert v.plan_info() == plan
+            v.set_state(re, im)
+            nl = v.vel_nonlinear()
+            v.set_state(re, im)
+            v.vel_TS(5)
+            out[plan] = (nl, v.get_state(), v.vel_energy())
+            v.close()
+    finally:
+        mw.autotune(0)
+    (na, sa, ea), (nb, sb, eb) = out[2], out[34]
+    nerr = max(np.abs(na[i] - nb[i]).max() for i in range(2)) / max(np.abs(na[0]).max(), np.abs(na[1]).max())
+    serr = rel_l2(sb, sa)
+    print(f"m6_2048 phase-split vs phase-sequential: nonlinear {nerr:.2e}, 5 steps {serr:.2e}, "
+          f"energy {abs(eb - ea) / ea:.2e}", flush=True)
+    assert nerr <= 1e-13 and serr <= 1e-13 and abs(eb - ea) <= 1e-13 * ea
