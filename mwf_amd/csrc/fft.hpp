@@ -129,39 +129,48 @@ template <int L, int Ns> constexpr int radix_at() { return (L / Ns) >= 8 ? 8 : (
 // apart cover all eight 16-byte bank groups, the loads of one butterfly share
 // one address, and the values are those of the natural table (the third
 // entry of a radix-4 triple is filler).
+//
+// Width PW = 7 stores all seven powers w^e .. w^{7e} of a radix-8 butterfly
+// instead (lanes 7 entries apart: 7 is odd, so still every bank group): the
+// butterfly then loads four more values and forms no products of twiddles,
+// 16 fewer VALU operations.  The lane-group transforms use it; the block-wide
+// passes keep the triples, whose tables are smaller.
 // ---------------------------------------------------------------------------
-template <int R> constexpr int ptw_width() { return R == 2 ? 1 : 3; }
-template <int L, int Ns = 1> constexpr int ptw_size()
+template <int R, int PW = 3> constexpr int ptw_width() { return R == 2 ? 1 : (R == 8 && PW == 7 ? 7 : 3); }
+template <int L, int Ns = 1, int PW = 3> constexpr int ptw_size()
 {
     if constexpr (Ns >= L) return 0;
-    else return (Ns > 1 ? ptw_width<radix_at<L, Ns>()>() * Ns : 0) + ptw_size<L, Ns * radix_at<L, Ns>()>();
+    else
+        return (Ns > 1 ? ptw_width<radix_at<L, Ns>(), PW>() * Ns : 0) + ptw_size<L, Ns * radix_at<L, Ns>(), PW>();
 }
-template <int L, int NsT, int Ns = 1> constexpr int ptw_off()
+template <int L, int NsT, int Ns = 1, int PW = 3> constexpr int ptw_off()
 {
     if constexpr (Ns >= NsT) return 0;
-    else return (Ns > 1 ? ptw_width<radix_at<L, Ns>()>() * Ns : 0) + ptw_off<L, NsT, Ns * radix_at<L, Ns>()>();
+    else
+        return (Ns > 1 ? ptw_width<radix_at<L, Ns>(), PW>() * Ns : 0) + ptw_off<L, NsT, Ns * radix_at<L, Ns>(), PW>();
 }
 
-// dst[i] for i = t, t + T, ... < ptw_size<L>() from the natural table tw (L entries).
-template <typename CT, int LOG, int Ns = 1>
+// dst[i] for i = t, t + T, ... < ptw_size<L, 1, PW>() from the natural table tw (L entries).
+template <typename CT, int LOG, int Ns = 1, int PW = 3>
 __device__ __forceinline__ void ptw_fill(CT* __restrict__ dst, const CT* __restrict__ tw, int t, int T)
 {
     constexpr int L = 1 << LOG;
     if constexpr (Ns < L) {
         constexpr int R = radix_at<L, Ns>();
         if constexpr (Ns > 1) {
-            constexpr int off = ptw_off<L, Ns>(), W = ptw_width<R>();
+            constexpr int off = ptw_off<L, Ns, 1, PW>(), W = ptw_width<R, PW>();
             for (int i = t; i < W * Ns; i += T) {
                 const int k = i / W, m = i % W;
-                dst[off + i] = tw[((k * (L / (Ns * R))) << m) & (L - 1)];
+                const int e = k * (L / (Ns * R));
+                dst[off + i] = tw[(W == 7 ? e * (m + 1) : e << m) & (L - 1)];
             }
         }
-        ptw_fill<CT, LOG, Ns * R>(dst, tw, t, T);
+        ptw_fill<CT, LOG, Ns * R, PW>(dst, tw, t, T);
     }
 }
 
-// x[r] *= w^r from the pass table triple at p (conjugated for SIGN < 0)
-template <int R, int SIGN, typename CT>
+// x[r] *= w^r from the pass table entries at p (conjugated for SIGN < 0)
+template <int R, int SIGN, int PW = 3, typename CT>
 __device__ __forceinline__ void twiddle_p(CT* x, const CT* __restrict__ p)
 {
     auto ld = [&](int i) {
@@ -171,6 +180,9 @@ __device__ __forceinline__ void twiddle_p(CT* x, const CT* __restrict__ p)
     };
     if constexpr (R == 2) {
         x[1] = cmul(x[1], ld(0));
+    } else if constexpr (R == 8 && PW == 7) {
+#pragma unroll
+        for (int r = 1; r < 8; r++) x[r] = cmul(x[r], ld(r - 1));
     } else if constexpr (R == 4) {
         const CT w1 = ld(0), w2 = ld(1);
         x[1] = cmul(x[1], w1);
@@ -191,10 +203,11 @@ __device__ __forceinline__ void twiddle_p(CT* x, const CT* __restrict__ p)
 
 // Twiddles of butterfly j in the pass (L, R, Ns): from the natural table, or
 // with PTW from the pass table.
-template <int L, int R, int Ns, int SIGN, bool PTW, typename CT>
+template <int L, int R, int Ns, int SIGN, bool PTW, int PW = 3, typename CT>
 __device__ __forceinline__ void twiddle_j(CT* x, const CT* __restrict__ tw, int j)
 {
-    if constexpr (PTW) twiddle_p<R, SIGN>(x, tw + ptw_off<L, Ns>() + ptw_width<R>() * (j & (Ns - 1)));
+    if constexpr (PTW)
+        twiddle_p<R, SIGN, PW>(x, tw + ptw_off<L, Ns, 1, PW>() + ptw_width<R, PW>() * (j & (Ns - 1)));
     else twiddle<R, SIGN>(x, tw, (j & (Ns - 1)) * (L / (Ns * R)));
 }
 
@@ -295,7 +308,7 @@ template <int L> constexpr int group_lanes() { return L / 8 < 64 ? L / 8 : 64; }
 // xo: a constant XORed into every slot of the group's region (a
 // per-sequence swizzle of the caller's layout; any xo < 16 keeps the access
 // patterns of the passes free of bank conflicts).
-template <typename CT, int LOG, int SIGN, bool PTW = false, int Ns = 1, int PASS = 0>
+template <typename CT, int LOG, int SIGN, bool PTW = false, int PW = 3, int Ns = 1, int PASS = 0>
 __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, const CT* __restrict__ tw, int xo = 0)
 {
     constexpr int L = 1 << LOG;
@@ -320,7 +333,7 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
 #pragma unroll
         for (int i = 0; i < NBL; i++) {
             const int j = gl + G * i;
-            if constexpr (Ns > 1) twiddle_j<L, R, Ns, SIGN, PTW>(x[i], tw, j);
+            if constexpr (Ns > 1) twiddle_j<L, R, Ns, SIGN, PTW, PW>(x[i], tw, j);
             dftR<R, SIGN>(x[i]);
         }
         if constexpr (LAST) {
@@ -338,7 +351,7 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
                 static_for<0, R>([&](auto ri) { swz<decltype(ri)::value * Ns>(reg, po) = x[i][ri]; });
             }
             wave_lds_sync();
-            group_fft<CT, LOG, SIGN, PTW, Ns * R, PASS + 1>(v, reg, gl, tw, xo);
+            group_fft<CT, LOG, SIGN, PTW, PW, Ns * R, PASS + 1>(v, reg, gl, tw, xo);
         }
     }
 }

@@ -1404,9 +1404,9 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     constexpr bool XREG = !INV_ONLY && LOGMM < 10;
     extern __shared__ unsigned char smem_raw[];
     CT* reg = reinterpret_cast<CT*>(smem_raw); // [XS/G sequences][MM], swizzled
-    CT* twMl = reg + (XS / G) * MM;            // pass twiddles of the MM-point transforms
-    static_assert(ptw_size<MM>() % 2 == 0, "twPl must stay 16-byte aligned");
-    double2* twPl = reinterpret_cast<double2*>(twMl + ptw_size<MM>()); // exp(2 pi i j/P), j <= 2 MM
+    CT* twMl = reg + (XS / G) * MM;            // pass twiddles of the MM-point transforms (seven-entry)
+    static_assert(ptw_size<MM, 1, 7>() % 2 == 0, "twPl must stay 16-byte aligned");
+    double2* twPl = reinterpret_cast<double2*>(twMl + ptw_size<MM, 1, 7>()); // exp(2 pi i j/P), j <= 2 MM
     constexpr bool ASYNC = xstep_async<R>() && !INV_ONLY;
     CT* stage = reinterpret_cast<CT*>(twPl + 2 * MM + 1);               // [3 XTILE] next NL panel
 
@@ -1435,7 +1435,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     double2 psi[IPT][2], phi[IPT][2];
     double2 xd[FLUX ? IPT : 1][2];
 
-    ptw_fill<CT, LOGMM>(twMl, twMM, tid, XS);
+    ptw_fill<CT, LOGMM, 1, 7>(twMl, twMM, tid, XS);
     for (int i = tid; i <= 2 * MM; i += XS) twPl[i] = twP[i];
     // the tables are read by other threads; the inverse-only form reads twPl
     // before its first transform barrier
@@ -1494,7 +1494,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                 CT v[PL];
                 const int pg = pidx(gl) ^ sxo;
                 static_for<0, PL>([&](auto qi) { v[qi] = swz<decltype(qi)::value * G>(myreg, pg); });
-                group_fft<CT, LOGMM, -1, true>(v, myreg, gl, twMl, sxo);
+                group_fft<CT, LOGMM, -1, true, 7>(v, myreg, gl, twMl, sxo);
                 static_for<0, PL>([&](auto qi) { swz<decltype(qi)::value * G>(myreg, pg) = v[qi]; });
             }
             __syncthreads();
@@ -1646,7 +1646,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             CT v[PL];
             const int pg = pidx(gl) ^ sxo;
             static_for<0, PL>([&](auto qi) { v[qi] = swz<decltype(qi)::value * G>(myreg, pg); });
-            group_fft<CT, LOGMM, +1, true>(v, myreg, gl, twMl, sxo);
+            group_fft<CT, LOGMM, +1, true, 7>(v, myreg, gl, twMl, sxo);
             static_for<0, PL>([&](auto qi) { swz<decltype(qi)::value * G>(myreg, pg) = v[qi]; });
         }
         __syncthreads();

@@ -37,7 +37,7 @@ static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream
 template <typename R, int LOGMM> static size_t xstep_lds()
 {
     constexpr size_t MM = (size_t)1 << LOGMM;
-    return ((xs_of<LOGMM>() / group_lanes<(1 << LOGMM)>()) * MM + ptw_size<(1 << LOGMM)>()) * sizeof(cplx_t<R>) +
+    return ((xs_of<LOGMM>() / group_lanes<(1 << LOGMM)>()) * MM + ptw_size<(1 << LOGMM), 1, 7>()) * sizeof(cplx_t<R>) +
            (2 * MM + 1) * sizeof(double2) + xstep_stage_bytes<R>();
 }
 

@@ -31,12 +31,19 @@
 
 template <int LOGNN> constexpr int zc_threads() { return (1 << LOGNN) > ZT ? (1 << LOGNN) : ZT; }
 
-// in-place transform buffer, then the pass twiddle table of the NN-point
+// Lane-group transforms where their registers fit (NN <= 512, and FP32 or
+// the two-buffer form), with seven-entry pass tables; otherwise block-wide
+// Stockham passes (ip_fft) with three-entry ones.
+template <typename R, int LOGNN, bool PP> constexpr bool zc_wave() { return LOGNN <= 9 && (PP || sizeof(R) == 4); }
+template <typename R, int LOGNN, bool PP> constexpr int zc_pw() { return zc_wave<R, LOGNN, PP>() ? 7 : 3; }
+
+// transform buffer(s), then the pass twiddle table of the NN-point
 // transforms and the first NN entries of exp(2 pi i j/Q) (twq2 extends them)
 template <typename R, int LOGNN, int ZP, bool PP> constexpr size_t zc_lds_bytes()
 {
     constexpr int NN = 1 << LOGNN, XB = zc_threads<LOGNN>() / NN;
-    return sizeof(cplx_t<R>) * ((PP ? 2 : 1) * (size_t)XB * ZP * 3 * NN + ptw_size<NN>() + (size_t)NN);
+    return sizeof(cplx_t<R>) *
+           ((PP ? 2 : 1) * (size_t)XB * ZP * 3 * NN + ptw_size<NN, 1, zc_pw<R, LOGNN, PP>()>() + (size_t)NN);
 }
 
 template <typename R, int K0, int LOGNN, int ZP, int WAVES, bool PP>
@@ -66,14 +73,15 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     extern __shared__ unsigned char smem_raw[];
     CT* bufA = reinterpret_cast<CT*>(smem_raw);       // [SEQ][NN]: the sequences of a batch
     CT* bufB = bufA + (PP ? SEQ * NN : 0);            // [SEQ][NN]: those of the next batch (PP)
-    CT* twNl = bufB + SEQ * NN;                       // [ptw_size<NN>()] pass twiddles
+    constexpr int PW = zc_pw<R, LOGNN, PP>();
+    CT* twNl = bufB + SEQ * NN;                       // [ptw_size<NN, 1, PW>()] pass twiddles
     // buffer of batch k (inverse batches first, then forward ones)
     auto buf_of = [&](int k) { return (PP && (k & 1)) ? bufB : bufA; };
     // transforms of the first S sequences of b, in place, natural order in
     // and out; ends with a block barrier.  Lane-group transforms where their
     // registers fit (NN <= 512, and FP32 or the two-wave form); otherwise
     // Stockham passes over the whole block (ip_fft, a barrier per half-pass).
-    constexpr bool WAVE = LOGNN <= 9 && (PP || sizeof(R) == 4);
+    constexpr bool WAVE = zc_wave<R, LOGNN, PP>();
     auto transform = [&](auto sign, int S, CT* b, int t) {
         if constexpr (!WAVE) {
             ip_fft<CT, LOGNN, decltype(sign)::value, T, SEQ, true>(b, S, twNl, t);
@@ -86,13 +94,13 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             CT v[PL];
             const int pg = pidx(gl);
             static_for<0, PL>([&](auto qi) { v[qi] = swz<decltype(qi)::value * G>(sq, pg); });
-            group_fft<CT, LOGNN, decltype(sign)::value, true>(v, sq, gl, twNl);
+            group_fft<CT, LOGNN, decltype(sign)::value, true, PW>(v, sq, gl, twNl);
             static_for<0, PL>([&](auto qi) { swz<decltype(qi)::value * G>(sq, pg) = v[qi]; });
         }
         __syncthreads();
     };
     static_assert(SEQ * group_lanes<NN>() <= T, "one lane group per sequence");
-    CT* twQl = twNl + ptw_size<NN>();                 // [NN]: w^k, k < NN
+    CT* twQl = twNl + ptw_size<NN, 1, PW>();          // [NN]: w^k, k < NN
 
     const int tid = threadIdx.x;
     const int xl = tid >> LOGNN, r = tid & (NN - 1);
@@ -130,7 +138,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
 
     R vb[3][KK];
     prefetch(0);
-    ptw_fill<CT, LOGNN>(twNl, twNN, tid, T);
+    ptw_fill<CT, LOGNN, 1, PW>(twNl, twNN, tid, T);
     for (int i = tid; i < NN; i += T) twQl[i] = twQ[i];
     __syncthreads();
 
