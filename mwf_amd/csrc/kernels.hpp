@@ -1426,6 +1426,21 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     const bool seq_on = s < 3 * NT;
     CT* myreg = reg + s * MM;
     const int sxo = xsw<R>(s / 3, s % 3); // swizzle of this thread's group sequence
+    // Decimating transposes between a panel ([x][t], i = x NT + t, x = 3a + b)
+    // and the sequences (t, b).  When the panel is a whole tile (NT = NTC, the
+    // usual case) item q of thread tid, i = tid + XS q, keeps t and b and has
+    // a = a0 + G q (XS = 3 NTC G), so its slot is one per-thread base and a
+    // constant (swz); otherwise every item's indices are computed.
+    constexpr int NTC = XTILE >> LOGMM, LOGNTC = 11 - LOGMM;
+    static_assert((1 << LOGNTC) == NTC && XS == 3 * NTC * G, "whole-tile transposes");
+    const bool whole = NT == NTC;
+    auto tile_item = [&](int tidv, int& t0, int& xq, CT*& rb, int& pa) {
+        t0 = tidv & (NTC - 1);
+        xq = tidv >> LOGNTC;
+        const int a0 = xq / 3, b0 = xq - 3 * a0;
+        rb = reg + (t0 * 3 + b0) * MM;
+        pa = pidx(a0) ^ xsw<R>(t0, b0);
+    };
     const double ak1 = ak1_of(k, g.beta);
     const double ak2 = ak2_of(k, g.beta);
 
@@ -1466,13 +1481,23 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                 // for the transforms, then start the next line's panel
                 wait_vmem();
                 __syncthreads();
+                if (whole) {
+                    int t0, xq, pa;
+                    CT* rb;
+                    tile_item(tidv, t0, xq, rb, pa);
+                    static_for<0, LPT>([&](auto qi) {
+                        constexpr int q = decltype(qi)::value;
+                        swz<q * G>(rb, pa) = stage[tidv + XS * q];
+                    });
+                } else {
 #pragma unroll
-                for (int q = 0; q < LPT; q++) {
-                    const int i = tidv + XS * q;
-                    if (i >= nrows) continue;
-                    const int x = i >> g.lgNT, t = i & (NT - 1);
-                    const int a = x / 3, b = x - 3 * a;
-                    reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = stage[i];
+                    for (int q = 0; q < LPT; q++) {
+                        const int i = tidv + XS * q;
+                        if (i >= nrows) continue;
+                        const int x = i >> g.lgNT, t = i & (NT - 1);
+                        const int a = x / 3, b = x - 3 * a;
+                        reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = stage[i];
+                    }
                 }
                 __syncthreads();
                 int ln = li + 1;
@@ -1480,13 +1505,23 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                 if (ln < NLI) panel_to_lds<XS>(panel_of(ln), stage, nch, tid);
             } else {
                 const CT* pan = panel_of(li);
+                if (whole) {
+                    int t0, xq, pa;
+                    CT* rb;
+                    tile_item(tidv, t0, xq, rb, pa);
+                    static_for<0, LPT>([&](auto qi) {
+                        constexpr int q = decltype(qi)::value;
+                        swz<q * G>(rb, pa) = ld_stream(pan + tidv + XS * q);
+                    });
+                } else {
 #pragma unroll
-                for (int q = 0; q < LPT; q++) {
-                    const int i = tidv + XS * q;
-                    if (i >= nrows) continue;
-                    const int x = i >> g.lgNT, t = i & (NT - 1);
-                    const int a = x / 3, b = x - 3 * a;
-                    reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = ld_stream(pan + i);
+                    for (int q = 0; q < LPT; q++) {
+                        const int i = tidv + XS * q;
+                        if (i >= nrows) continue;
+                        const int x = i >> g.lgNT, t = i & (NT - 1);
+                        const int a = x / 3, b = x - 3 * a;
+                        reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = ld_stream(pan + i);
+                    }
                 }
                 __syncthreads();
             }
@@ -1652,13 +1687,25 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
         __syncthreads();
         CT* Ul = U + (size_t)line * g.NN + n0;
         const int ustride = g.UL * g.NN;
+        if (whole) {
+            int t0, xq, pa;
+            CT* rb;
+            tile_item(tidv, t0, xq, rb, pa);
+            CT* ub = Ul + (size_t)xq * ustride + t0;
+            const size_t ustep = (size_t)(3 * G) * ustride; // x advances 3 G per item
+            static_for<0, LPT>([&](auto qi) {
+                constexpr int q = decltype(qi)::value;
+                st_piece(ub + q * ustep, swz<q * G>(rb, pa), wide);
+            });
+        } else {
 #pragma unroll
-        for (int q = 0; q < LPT; q++) {
-            const int i = tidv + XS * q;
-            if (i >= nrows) continue;
-            const int x = i >> g.lgNT, t = i & (NT - 1);
-            const int a = x / 3, b = x - 3 * a;
-            st_piece(Ul + x * ustride + t, reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))], wide);
+            for (int q = 0; q < LPT; q++) {
+                const int i = tidv + XS * q;
+                if (i >= nrows) continue;
+                const int x = i >> g.lgNT, t = i & (NT - 1);
+                const int a = x / 3, b = x - 3 * a;
+                st_piece(Ul + x * ustride + t, reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))], wide);
+            }
         }
         __syncthreads();
     }

@@ -126,11 +126,13 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             const CT z{(R)0, (R)0};
             pf[f][0] = pf[f][1] = pf[f][2] = pf[f][3] = z;
             if (xg < g.P && pr < NPI) {
-                pf[f][0] = U[uidx(g, xg, 2 * pr, fr)];
-                pf[f][1] = U[uidx(g, xg, 2 * pr, r2)];
+                // U[x][line][n] with KK lines of NN (uidx, with the flux form's sizes)
+                const CT* u0 = U + ((size_t)(xg * KK + 2 * pr) << LOGNN);
+                pf[f][0] = u0[fr];
+                pf[f][1] = u0[r2];
                 if (2 * pr + 1 < KK) {
-                    pf[f][2] = U[uidx(g, xg, 2 * pr + 1, fr)];
-                    pf[f][3] = U[uidx(g, xg, 2 * pr + 1, r2)];
+                    pf[f][2] = u0[NN + fr];
+                    pf[f][3] = u0[NN + r2];
                 }
             }
         }
@@ -199,6 +201,11 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         const int tid = opaque_tid();
         const int xl = tid >> LOGNN, r = tid & (NN - 1);
         CT* buf = buf_of(NBI + RD);
+        // this thread's element of NL line L is nlr[L * nls] (nidx with the
+        // flux form's 2 KK lines): one base per batch instead of the full
+        // index per store
+        CT* nlr = NL + ((((size_t)(r >> g.lgNT) * (2 * KK) * g.P) + (x0 + xl)) << g.lgNT) + (r & (g.NTx - 1));
+        const size_t nls = (size_t)g.P << g.lgNT;
         // the base values pass through an empty asm per batch: products shared
         // by rows of different batches are formed again in each batch instead
         // of being computed once and held in registers across all of them
@@ -255,7 +262,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
                         if constexpr (FX::first(L) == q) acc[L] = term;
                         else acc[L] = cadd(acc[L], term);
                         if constexpr (FX::last(L) == q) {
-                            if (valid) st_piece(&NL[nidx(g, x, L, r)], acc[L], wide);
+                            if (valid) st_piece(nlr + L * nls, acc[L], wide);
                         }
                     });
                 }
