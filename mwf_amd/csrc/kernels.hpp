@@ -26,7 +26,8 @@
 #include "galerkin_gen.hpp"
 
 constexpr int XT = 512;     // threads per block of the x kernels
-constexpr int XTILE = 2048; // NTx * MM: x-kernel tile (complex values per line)
+constexpr int LOG_XTILE = 11;
+constexpr int XTILE = 1 << LOG_XTILE; // NTx * MM: x-kernel tile (complex values per line)
 constexpr int ZT = 512;     // threads per block of the z kernels (= XB * NN)
 constexpr int ZPB = 2;      // component pairs per z batch
 
@@ -1431,7 +1432,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     // usual case) item q of thread tid, i = tid + XS q, keeps t and b and has
     // a = a0 + G q (XS = 3 NTC G), so its slot is one per-thread base and a
     // constant (swz); otherwise every item's indices are computed.
-    constexpr int NTC = XTILE >> LOGMM, LOGNTC = 11 - LOGMM;
+    constexpr int NTC = XTILE >> LOGMM, LOGNTC = LOG_XTILE - LOGMM;
     static_assert((1 << LOGNTC) == NTC && XS == 3 * NTC * G, "whole-tile transposes");
     const bool whole = NT == NTC;
     auto tile_item = [&](int tidv, int& t0, int& xq, CT*& rb, int& pa) {
