@@ -327,14 +327,19 @@ def profile_case(name, steps, warmup, no_mixed, tune):
     print(json.dumps(out), flush=True)
 
 
-def traffic_summary():
-    """HBM bytes per k_z launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), if any."""
-    path = os.path.join(ROOT, "profiles", "k_z_m4_1024_fp64_traffic.json")
+def traffic_summary(kernel):
+    """HBM bytes per launch of one FP64 kernel of the M4 1024^2 step from the
+    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE scaled by the
+    copy-kernel calibration), if any."""
+    path = os.path.join(ROOT, "profiles", "k_zc_k_xstep_m4_1024_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel, {}).get("fp64")
+    if not k:
+        return None
+    return {"hbm_bytes_per_launch": (k["read_MB"] + k["write_MB"]) * 1e6, "source": d["source"]}
 
 
 def main():
@@ -412,8 +417,8 @@ def main():
         "plan": fp64["plan"],
         "finite": fp64["finite"],
     }
-    tr = traffic_summary()
-    if tr and tr.get("kernel") == dom_name:
+    tr = traffic_summary(dom_name)
+    if tr:
         line["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
         line["roofline"]["traffic_source"] = tr["source"]
     if mixed:
