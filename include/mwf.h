@@ -174,9 +174,9 @@ int mwf_autotune(int32_t mode);
  * (u.grad)u, with d/dx, d/dz taken in spectral space and d/dy moved onto the
  * wall-normal bases, equal to the advective form up to rounding; bit 4 (with
  * bit 3) = its z kernel compiled for two waves per SIMD instead of four;
- * bit 5 (advective form, NN = 1024; implies bit 1) = the phase-sequential z
- * kernel with the three fold phases of an x-row in three blocks, each writing
- * its own NL partial, which the x-forward kernel sums. */
+ * bit 5 (advective form, NN = 1024; implies bit 1 in the mixed form) = the
+ * phase-sequential z kernel with the three fold phases of an x-row in three
+ * blocks, each writing its own NL partial, which the x kernel sums. */
 int mwf_plan_info(mwf_handle *h, int32_t *choice);
 
 /* Wait for all queued work of the handle. */

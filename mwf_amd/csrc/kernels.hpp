@@ -1343,7 +1343,11 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
 // hold psi_k / phi_k at m = r and m = r - MM.  768 threads for MM <= 512,
 // 384 at MM = 1024.
 // ---------------------------------------------------------------------------
-template <int LOGMM> constexpr int xs_of() { return 3 * (XTILE >> LOGMM) * group_lanes<(1 << LOGMM)>(); }
+// Threads per block: one lane group per sequence, 3 XTILE/MM sequences; at MM
+// = 1024 (6 sequences) twice that, so that a thread holds 3 items instead of
+// 6 (the extra waves take items but no sequence).
+template <int LOGMM> constexpr int xs_mult() { return LOGMM >= 10 ? 2 : 1; }
+template <int LOGMM> constexpr int xs_of() { return xs_mult<LOGMM>() * 3 * (XTILE >> LOGMM) * group_lanes<(1 << LOGMM)>(); }
 
 // FP32 k_xstep streams the next NL panel straight into an LDS staging area
 // (global_load_lds, no registers) while the current line is transformed: the
@@ -1405,7 +1409,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     constexpr bool XREG = !INV_ONLY && LOGMM < 10;
     extern __shared__ unsigned char smem_raw[];
     CT* reg = reinterpret_cast<CT*>(smem_raw); // [XS/G sequences][MM], swizzled
-    CT* twMl = reg + (XS / G) * MM;            // pass twiddles of the MM-point transforms (seven-entry)
+    CT* twMl = reg + 3 * XTILE;                // pass twiddles of the MM-point transforms (seven-entry)
     static_assert(ptw_size<MM, 1, 7>() % 2 == 0, "twPl must stay 16-byte aligned");
     double2* twPl = reinterpret_cast<double2*>(twMl + ptw_size<MM, 1, 7>()); // exp(2 pi i j/P), j <= 2 MM
     constexpr bool ASYNC = xstep_async<R>() && !INV_ONLY;
@@ -1430,10 +1434,12 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     // Decimating transposes between a panel ([x][t], i = x NT + t, x = 3a + b)
     // and the sequences (t, b).  When the panel is a whole tile (NT = NTC, the
     // usual case) item q of thread tid, i = tid + XS q, keeps t and b and has
-    // a = a0 + G q (XS = 3 NTC G), so its slot is one per-thread base and a
-    // constant (swz); otherwise every item's indices are computed.
+    // a = a0 + DA q (DA = XS / (3 NTC), a power of two above a0), so its slot
+    // is one per-thread base and a constant (swz); otherwise every item's
+    // indices are computed.
     constexpr int NTC = XTILE >> LOGMM, LOGNTC = LOG_XTILE - LOGMM;
-    static_assert((1 << LOGNTC) == NTC && XS == 3 * NTC * G, "whole-tile transposes");
+    constexpr int DA = XS / (3 * NTC);
+    static_assert((1 << LOGNTC) == NTC && XS == 3 * NTC * DA && (DA & (DA - 1)) == 0, "whole-tile transposes");
     const bool whole = NT == NTC;
     auto tile_item = [&](int tidv, int& t0, int& xq, CT*& rb, int& pa) {
         t0 = tidv & (NTC - 1);
@@ -1488,7 +1494,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                     tile_item(tidv, t0, xq, rb, pa);
                     static_for<0, LPT>([&](auto qi) {
                         constexpr int q = decltype(qi)::value;
-                        swz<q * G>(rb, pa) = stage[tidv + XS * q];
+                        swz<q * DA>(rb, pa) = stage[tidv + XS * q];
                     });
                 } else {
 #pragma unroll
@@ -1506,13 +1512,21 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                 if (ln < NLI) panel_to_lds<XS>(panel_of(ln), stage, nch, tid);
             } else {
                 const CT* pan = panel_of(li);
+                // NL element, or the sum ((P0 + P1) + P2) / (PQ) of the three
+                // fold-phase partials of the phase-split z kernel
+                const size_t nlp = bt.nl_part;
+                auto nl_at = [&](const CT* p) {
+                    CT v = ld_stream(p);
+                    if (nlp) v = cscale(cadd(cadd(v, ld_stream(p + nlp)), ld_stream(p + 2 * nlp)), (R)g.scale);
+                    return v;
+                };
                 if (whole) {
                     int t0, xq, pa;
                     CT* rb;
                     tile_item(tidv, t0, xq, rb, pa);
                     static_for<0, LPT>([&](auto qi) {
                         constexpr int q = decltype(qi)::value;
-                        swz<q * G>(rb, pa) = ld_stream(pan + tidv + XS * q);
+                        swz<q * DA>(rb, pa) = nl_at(pan + tidv + XS * q);
                     });
                 } else {
 #pragma unroll
@@ -1521,7 +1535,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                         if (i >= nrows) continue;
                         const int x = i >> g.lgNT, t = i & (NT - 1);
                         const int a = x / 3, b = x - 3 * a;
-                        reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = ld_stream(pan + i);
+                        reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = nl_at(pan + i);
                     }
                 }
                 __syncthreads();
@@ -1693,10 +1707,10 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             CT* rb;
             tile_item(tidv, t0, xq, rb, pa);
             CT* ub = Ul + (size_t)xq * ustride + t0;
-            const size_t ustep = (size_t)(3 * G) * ustride; // x advances 3 G per item
+            const size_t ustep = (size_t)(3 * DA) * ustride; // x advances 3 DA per item
             static_for<0, LPT>([&](auto qi) {
                 constexpr int q = decltype(qi)::value;
-                st_piece(ub + q * ustep, swz<q * G>(rb, pa), wide);
+                st_piece(ub + q * ustep, swz<q * DA>(rb, pa), wide);
             });
         } else {
 #pragma unroll

@@ -262,7 +262,9 @@ struct SizeClass {
         // phase-split z kernel: three NL partials, summed by the split x kernels
         const bool ph3 = !flux && (ch & 32) && base.zf_ph3;
         if (ph3) plan.zf = base.zf_ph3;
-        if ((ch & 2) || ph3) plan.xstep = nullptr;
+        // the fused x kernel sums the partials in FP64; FP32 stages whole NL
+        // panels into LDS and takes them with the split x kernels
+        if ((ch & 2) || (ph3 && prec == MWF_MIXED)) plan.xstep = nullptr;
         bt.nl_part = ph3 ? bt.nl_stride * nmem : 0;
         const int UL = flux ? g0.KK : 2 * g0.KK, NLL = flux ? 2 * g0.KK : g0.KK;
         if (g0.UL != UL) {
@@ -302,7 +304,10 @@ struct SizeClass {
         if (base.zf_alt2) zc.push_back(4);
         if (base.zc) zc.push_back(8);
         if (base.zc_alt) zc.push_back(8 | 16);
-        if (base.zf_ph3) cand.push_back(32 | 2);
+        if (base.zf_ph3) {
+            cand.push_back(32 | 2);
+            if (prec != MWF_MIXED && base.xstep) cand.push_back(32);
+        }
         for (int z : zc) {
             if ((z & 8) ? (bool)base.xstep_f : (bool)base.xstep) cand.push_back(z);
             cand.push_back(z | 2);

@@ -37,7 +37,7 @@ static hipError_t launch_xfwd(const Batch& bt, int nmem, const Geo& g, hipStream
 template <typename R, int LOGMM> static size_t xstep_lds()
 {
     constexpr size_t MM = (size_t)1 << LOGMM;
-    return ((xs_of<LOGMM>() / group_lanes<(1 << LOGMM)>()) * MM + ptw_size<(1 << LOGMM), 1, 7>()) * sizeof(cplx_t<R>) +
+    return (3 * (size_t)XTILE + ptw_size<(1 << LOGMM), 1, 7>()) * sizeof(cplx_t<R>) +
            (2 * MM + 1) * sizeof(double2) + xstep_stage_bytes<R>();
 }
 
@@ -100,12 +100,9 @@ template <typename R> void pick_x(int lm, Plan& p)
         p.xfwd_f = launch_xfwd<R, 10, true>;
         p.sxinv = launch_sxinv<R, 10>;
         p.sxfwd = launch_sxfwd<R, 10>;
-        // the fused kernel at MM = 1024 pays off in FP64 (2.16 against 2.66 ms at
-        // M6 2048^2); in FP32 its 16-point lanes cost occupancy and it is slower
-        if constexpr (std::is_same<R, double>::value) {
-            p.xstep = launch_xstep<R, 10, false>;
-            p.xstep_f = launch_xstep<R, 10, true>;
-        }
+        // fused kernel at MM = 1024: a plan candidate next to the split kernels
+        p.xstep = launch_xstep<R, 10, false>;
+        p.xstep_f = launch_xstep<R, 10, true>;
         break;
     default: break;
     }
@@ -136,12 +133,10 @@ template <typename R> void prepare_x(int lm)
         set_lds(k_xfwd<R, 10, true>, x_lds<R, 10>());
         set_lds(k_sxinv<R, 10>, x_lds<R, 10>());
         set_lds(k_sxfwd<R, 10>, x_lds<R, 10>());
-        if constexpr (std::is_same<R, double>::value) {
-            set_lds(k_xstep<R, 10, false, false>, xstep_lds<R, 10>());
-            set_lds(k_xstep<R, 10, true, false>, xstep_lds<R, 10>());
-            set_lds(k_xstep<R, 10, false, true>, xstep_lds<R, 10>());
-            set_lds(k_xstep<R, 10, true, true>, xstep_lds<R, 10>());
-        }
+        set_lds(k_xstep<R, 10, false, false>, xstep_lds<R, 10>());
+        set_lds(k_xstep<R, 10, true, false>, xstep_lds<R, 10>());
+        set_lds(k_xstep<R, 10, false, true>, xstep_lds<R, 10>());
+        set_lds(k_xstep<R, 10, true, true>, xstep_lds<R, 10>());
         break;
     default: break;
     }

@@ -53,7 +53,7 @@ def test_timed_choice_is_cached(tuning):
     p = mw.make_params(6, 64, 512, 16 * math.pi, 128 * math.pi, 200.0, 0.02)
     a = mw.Velocity(p)
     b = mw.Velocity(p)
-    assert a.plan_info() in (0, 1, 2, 3, 4, 6, 8, 10, 24, 26, 34)
+    assert a.plan_info() in (0, 1, 2, 3, 4, 6, 8, 10, 24, 26, 32, 34)
     assert b.plan_info() == a.plan_info()
     # the tuner leaves a zero state and a forward-Euler first step
     re, im = a.get_state()
@@ -118,4 +118,27 @@ def test_phase_split_z_kernel_matches_oracle(tuning, k0, mm, nn, mixed):
           f"20 steps rel_l2 {err:.2e}, energy {abs(e - eref) / eref:.2e}", flush=True)
     assert nerr <= (1e-5 if mixed else 1e-12)
     assert err <= bar and abs(e - eref) <= bar * eref
+    v.close()
+
+
+@pytest.mark.parametrize("k0,mm,nn", [(4, 16, 1024), (6, 32, 1024)])
+def test_phase_split_z_kernel_fused_x_matches_oracle(tuning, k0, mm, nn):
+    """Plan bit 32 alone in FP64: the phase-split z kernel with the fused x
+    kernel summing the three NL partials, 20 steps against the oracle."""
+    tuning(2 + mw.PLAN_ZB_SPLIT)
+    p = mw.make_params(k0, mm, nn, mm * math.pi / 4, nn * math.pi / 4, 200.0, 0.02)
+    v = mw.Velocity(p)
+    assert v.plan_info() == mw.PLAN_ZB_SPLIT
+    o = oracle.Oracle(k0, mm, nn, p.alpha, p.gamma, p.re, p.dt, drag=p.drag, vdrag=p.vdrag,
+                      hypo=p.hypo, theta=p.theta, dragall=bool(p.dragall))
+    re, im = mw.ic_gaussian(p, 5, 0.2, 4.0)
+    v.set_state(re, im)
+    o.set_state(re, im)
+    v.vel_TS(20)
+    o.step(20)
+    err = rel_l2(v.get_state(), o.get_state())
+    e, eref = v.vel_energy(), o.energy()
+    print(f"phase-split z + fused x k0={k0} mm={mm} nn={nn}: 20 steps rel_l2 {err:.2e}, "
+          f"energy {abs(e - eref) / eref:.2e}", flush=True)
+    assert err <= 1e-12 and abs(e - eref) <= 1e-12 * eref
     v.close()

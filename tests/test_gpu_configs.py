@@ -148,15 +148,16 @@ def test_mixed_energy_within_1e5(name, steps):
 
 
 def test_m6_2048_phase_split_matches_phase_sequential():
-    """M6 2048^2 (the k_zb size), FP64: the phase-split z kernel (plan bits
-    32 | 2) against the phase-sequential one (pinned to the oracle at this size
-    by test_m6_2048_two_steps): nonlinear term and 5 steps, to rounding."""
+    """M6 2048^2 (the k_zb size), FP64: the phase-split z kernel with the split
+    (plan 34) and the fused (plan 32) x kernels against the phase-sequential
+    one (pinned to the oracle at this size by test_m6_2048_two_steps):
+    nonlinear term and 5 steps, to rounding."""
     p = cases.params("m6_2048")
     seed, scl, k0ic = cases.ic_of("m6_2048")
     re, im = mw.ic_gaussian(p, seed, scl, k0ic)
     out = {}
     try:
-        for plan in (2, 34):
+        for plan in (2, 34, 32):
             mw.autotune(2 + plan)
             v = mw.Velocity(p)
             assert v.plan_info() == plan
@@ -168,9 +169,11 @@ def test_m6_2048_phase_split_matches_phase_sequential():
             v.close()
     finally:
         mw.autotune(0)
-    (na, sa, ea), (nb, sb, eb) = out[2], out[34]
-    nerr = max(np.abs(na[i] - nb[i]).max() for i in range(2)) / max(np.abs(na[0]).max(), np.abs(na[1]).max())
-    serr = rel_l2(sb, sa)
-    print(f"m6_2048 phase-split vs phase-sequential: nonlinear {nerr:.2e}, 5 steps {serr:.2e}, "
-          f"energy {abs(eb - ea) / ea:.2e}", flush=True)
-    assert nerr <= 1e-13 and serr <= 1e-13 and abs(eb - ea) <= 1e-13 * ea
+    na, sa, ea = out[2]
+    for plan in (34, 32):
+        nb, sb, eb = out[plan]
+        nerr = max(np.abs(na[i] - nb[i]).max() for i in range(2)) / max(np.abs(na[0]).max(), np.abs(na[1]).max())
+        serr = rel_l2(sb, sa)
+        print(f"m6_2048 phase-split (plan {plan}) vs phase-sequential: nonlinear {nerr:.2e}, 5 steps {serr:.2e}, "
+              f"energy {abs(eb - ea) / ea:.2e}", flush=True)
+        assert nerr <= 1e-13 and serr <= 1e-13 and abs(eb - ea) <= 1e-13 * ea
