@@ -1358,16 +1358,13 @@ template <typename R> constexpr bool xstep_async() { return sizeof(R) == 4; }
 // Sequence (t, b) of the x kernels' transform buffer holds element a at slot
 // pidx(a) ^ xsw(t, b).  The decimating transposes between panels ([x][t],
 // x = 3a + b) and sequences have up to 16 lanes of one bank phase on one a;
-// in FP32 this per-sequence XOR sends them to distinct banks (conflict-free
-// at MM <= 256, at most 2-way at 512 and 1024, from 6- to 16-way
-// unswizzled), while accesses within one sequence keep their conflict-free
-// patterns.  FP64 keeps the plain layout: there the XOR ((t * 5) ^ b) & 7
-// would be as effective, but its index registers push k_xstep from 17 to 25
-// spilled VGPRs at MM = 512, which costs more (4 %) than the conflicts.
+// this per-sequence XOR sends them to distinct banks (conflict-free at MM <=
+// 256, at most 2-way at 512 and 1024, from 6- to 16-way unswizzled), while
+// accesses within one sequence keep their conflict-free patterns.
 template <typename R> __device__ __forceinline__ int xsw(int t, int b)
 {
     if constexpr (sizeof(R) == 4) return ((t * 9) ^ (b * 3)) & 15; // 8-byte slots, 16 per bank phase
-    else return 0;
+    else return ((t * 5) ^ b) & 7;                                 // 16-byte slots, 8 per bank phase
 }
 template <typename R> constexpr size_t xstep_stage_bytes() { return xstep_async<R>() ? 3 * (size_t)XTILE * sizeof(cplx_t<R>) : 0; }
 
