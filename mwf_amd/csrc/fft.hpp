@@ -40,6 +40,25 @@ template <int C, typename CT> __device__ __forceinline__ CT& swz(CT* base, int p
     return (base + (pa ^ LO))[HI];
 }
 
+// Padded layout (shift S > 0): element i at i + (i >> S), one spare slot per
+// 2^S, with sequences lspan<L, S>() slots apart.  The map is additive over
+// disjoint bit fields (no carries), so every butterfly address is the lane
+// part's slot plus a constant: no per-access index arithmetic at all.  A CPU
+// model of the lane-group patterns puts S = 3 at 1.02-way bank conflicts on
+// average for 16-byte values and S = 4 at 1.12-way for 8-byte ones.  S = 0
+// is the XOR swizzle above.
+template <int S> __host__ __device__ constexpr int lslot(int i)
+{
+    if constexpr (S == 0) return pidx(i);
+    else return i + (i >> S);
+}
+template <int C, int S, typename CT> __device__ __forceinline__ CT& lsw(CT* base, int pa)
+{
+    if constexpr (S == 0) return swz<C>(base, pa);
+    else return base[pa + lslot<S>(C)];
+}
+template <int L, int S> constexpr int lspan() { return S == 0 ? L : L + (L >> S); }
+
 template <int SIGN, typename CT> __device__ __forceinline__ void dft2(CT* x)
 {
     CT a = x[0], b = x[1];
@@ -307,8 +326,9 @@ template <int L> constexpr int group_lanes() { return L / 8 < 64 ? L / 8 : 64; }
 
 // xo: a constant XORed into every slot of the group's region (a
 // per-sequence swizzle of the caller's layout; any xo < 16 keeps the access
-// patterns of the passes free of bank conflicts).
-template <typename CT, int LOG, int SIGN, bool PTW = false, int PW = 3, int Ns = 1, int PASS = 0>
+// patterns of the passes free of bank conflicts).  LS: the region's layout
+// (lslot; 0 = XOR swizzle, xo must be 0 otherwise).
+template <typename CT, int LOG, int SIGN, bool PTW = false, int PW = 3, int Ns = 1, int PASS = 0, int LS = 0>
 __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, const CT* __restrict__ tw, int xo = 0)
 {
     constexpr int L = 1 << LOG;
@@ -325,8 +345,8 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
 #pragma unroll
                 for (int r = 0; r < R; r++) x[i][r] = v[i + r * NBL];
             } else {
-                const int pj = pidx(j) ^ xo;
-                static_for<0, R>([&](auto ri) { x[i][ri] = swz<decltype(ri)::value * (L / R)>(reg, pj); });
+                const int pj = lslot<LS>(j) ^ xo;
+                static_for<0, R>([&](auto ri) { x[i][ri] = lsw<decltype(ri)::value * (L / R), LS>(reg, pj); });
             }
         }
         if constexpr (PASS > 0) wave_lds_sync();
@@ -347,11 +367,11 @@ __device__ __forceinline__ void group_fft(CT* v, CT* __restrict__ reg, int gl, c
             for (int i = 0; i < NBL; i++) {
                 const int j = gl + G * i;
                 const int k = j & (Ns - 1);
-                const int po = pidx((j - k) * R + k) ^ xo;
-                static_for<0, R>([&](auto ri) { swz<decltype(ri)::value * Ns>(reg, po) = x[i][ri]; });
+                const int po = lslot<LS>((j - k) * R + k) ^ xo;
+                static_for<0, R>([&](auto ri) { lsw<decltype(ri)::value * Ns, LS>(reg, po) = x[i][ri]; });
             }
             wave_lds_sync();
-            group_fft<CT, LOG, SIGN, PTW, PW, Ns * R, PASS + 1>(v, reg, gl, tw, xo);
+            group_fft<CT, LOG, SIGN, PTW, PW, Ns * R, PASS + 1, LS>(v, reg, gl, tw, xo);
         }
     }
 }

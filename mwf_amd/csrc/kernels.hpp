@@ -349,19 +349,19 @@ __device__ __forceinline__ cplx_t<R> zrow(const Geo& g, const cplx_t<R>* __restr
 // (fr = 0: the DC bin with its imaginary part dropped, and the self-mirrored
 // NN/2 bin, Y = 2 Re(X w^{(NN/2) b})).
 // wr = w^{fr b} and wm = w^{(NN-fr) b} (fr = 0: wm = w^{(NN/2) b}); unused for b = 0.
-template <typename R>
+template <typename R, int LS = 0>
 __device__ __forceinline__ void zfold_bw(cplx_t<R>* __restrict__ seq, int b, int fr, int NN, cplx_t<R> a1,
                                          cplx_t<R> b1, cplx_t<R> a2, cplx_t<R> b2, cplx_t<R> wr, cplx_t<R> wm)
 {
     using CT = cplx_t<R>;
     const int H = NN / 2;
     if (fr == 0) {
-        seq[pidx(0)] = CT{a1.x, a2.x};
+        seq[lslot<LS>(0)] = CT{a1.x, a2.x};
         if (b == 0) {
-            seq[pidx(H)] = CT{(R)2 * b1.x, (R)2 * b2.x};
+            seq[lslot<LS>(H)] = CT{(R)2 * b1.x, (R)2 * b2.x};
         } else {
             const CT p1 = cmul(b1, wm), p2 = cmul(b2, wm);
-            seq[pidx(H)] = CT{(R)2 * p1.x, (R)2 * p2.x};
+            seq[lslot<LS>(H)] = CT{(R)2 * p1.x, (R)2 * p2.x};
         }
         return;
     }
@@ -377,8 +377,8 @@ __device__ __forceinline__ void zfold_bw(cplx_t<R>* __restrict__ seq, int b, int
     // row 1: Y[fr] = P1a + conj(P2a), Y[fm] = P2a + conj(P1a); row 2 likewise
     const CT y1r = CT{P1a.x + P2a.x, P1a.y - P2a.y}, y1m = CT{P2a.x + P1a.x, P2a.y - P1a.y};
     const CT y2r = CT{P1b.x + P2b.x, P1b.y - P2b.y}, y2m = CT{P2b.x + P1b.x, P2b.y - P1b.y};
-    seq[pidx(fr)] = CT{y1r.x - y2r.y, y1r.y + y2r.x};
-    seq[pidx(fm)] = CT{y1m.x - y2m.y, y1m.y + y2m.x};
+    seq[lslot<LS>(fr)] = CT{y1r.x - y2r.y, y1r.y + y2r.x};
+    seq[lslot<LS>(fm)] = CT{y1m.x - y2m.y, y1m.y + y2m.x};
 }
 
 template <typename R>

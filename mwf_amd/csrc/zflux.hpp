@@ -36,14 +36,19 @@ template <int LOGNN> constexpr int zc_threads() { return (1 << LOGNN) > ZT ? (1 
 // Stockham passes (ip_fft) with three-entry ones.
 template <typename R, int LOGNN, bool PP> constexpr bool zc_wave() { return LOGNN <= 9 && (PP || sizeof(R) == 4); }
 template <typename R, int LOGNN, bool PP> constexpr int zc_pw() { return zc_wave<R, LOGNN, PP>() ? 7 : 3; }
+// LDS layout of the sequences: padded (lslot shift 4) for the FP32 lane-group
+// transforms, XOR-swizzled otherwise (padding with shift 3 made FP64 k_zc 2 %
+// slower at M4 1024^2)
+template <typename R, int LOGNN, bool PP> constexpr int zc_ls() { return zc_wave<R, LOGNN, PP>() && sizeof(R) == 4 ? 4 : 0; }
 
 // transform buffer(s), then the pass twiddle table of the NN-point
 // transforms and the first NN entries of exp(2 pi i j/Q) (twq2 extends them)
 template <typename R, int LOGNN, int ZP, bool PP> constexpr size_t zc_lds_bytes()
 {
     constexpr int NN = 1 << LOGNN, XB = zc_threads<LOGNN>() / NN;
+    constexpr int LPZ = lspan<NN, zc_ls<R, LOGNN, PP>()>();
     return sizeof(cplx_t<R>) *
-           ((PP ? 2 : 1) * (size_t)XB * ZP * 3 * NN + ptw_size<NN, 1, zc_pw<R, LOGNN, PP>()>() + (size_t)NN);
+           ((PP ? 2 : 1) * (size_t)XB * ZP * 3 * LPZ + ptw_size<NN, 1, zc_pw<R, LOGNN, PP>()>() + (size_t)NN);
 }
 
 template <typename R, int K0, int LOGNN, int ZP, int WAVES, bool PP>
@@ -71,10 +76,11 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     constexpr int FT = XB * ZP * H / T;               // fold tasks per thread
     static_assert(FT * T == XB * ZP * H, "fold tasks must tile the block");
     extern __shared__ unsigned char smem_raw[];
-    CT* bufA = reinterpret_cast<CT*>(smem_raw);       // [SEQ][NN]: the sequences of a batch
-    CT* bufB = bufA + (PP ? SEQ * NN : 0);            // [SEQ][NN]: those of the next batch (PP)
+    constexpr int LS = zc_ls<R, LOGNN, PP>(), LPZ = lspan<NN, LS>(); // sequence layout and span
+    CT* bufA = reinterpret_cast<CT*>(smem_raw);       // [SEQ][LPZ]: the sequences of a batch
+    CT* bufB = bufA + (PP ? SEQ * LPZ : 0);           // [SEQ][LPZ]: those of the next batch (PP)
     constexpr int PW = zc_pw<R, LOGNN, PP>();
-    CT* twNl = bufB + SEQ * NN;                       // [ptw_size<NN, 1, PW>()] pass twiddles
+    CT* twNl = bufB + SEQ * LPZ;                      // [ptw_size<NN, 1, PW>()] pass twiddles
     // buffer of batch k (inverse batches first, then forward ones)
     auto buf_of = [&](int k) { return (PP && (k & 1)) ? bufB : bufA; };
     // transforms of the first S sequences of b, in place, natural order in
@@ -90,12 +96,12 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         constexpr int G = group_lanes<NN>(), PL = NN / G;
         const int gi = t / G, gl = t % G;
         if (gi < S) {
-            CT* sq = b + gi * NN;
+            CT* sq = b + gi * LPZ;
             CT v[PL];
-            const int pg = pidx(gl);
-            static_for<0, PL>([&](auto qi) { v[qi] = swz<decltype(qi)::value * G>(sq, pg); });
-            group_fft<CT, LOGNN, decltype(sign)::value, true, PW>(v, sq, gl, twNl);
-            static_for<0, PL>([&](auto qi) { swz<decltype(qi)::value * G>(sq, pg) = v[qi]; });
+            const int pg = lslot<LS>(gl);
+            static_for<0, PL>([&](auto qi) { v[qi] = lsw<decltype(qi)::value * G, LS>(sq, pg); });
+            group_fft<CT, LOGNN, decltype(sign)::value, true, PW, 1, 0, LS>(v, sq, gl, twNl);
+            static_for<0, PL>([&](auto qi) { lsw<decltype(qi)::value * G, LS>(sq, pg) = v[qi]; });
         }
         __syncthreads();
     };
@@ -166,11 +172,11 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             const int fx = tau / (ZP * H), fp = (tau / H) % ZP, fr = tau % H;
             if (XB == 1 && fp >= NV) continue;
             const int fm = fr == 0 ? H : NN - fr;
-            CT* seq = buf + ((fx * ZP + fp) * 3) * NN;
+            CT* seq = buf + ((fx * ZP + fp) * 3) * LPZ;
             const CT one{(R)1, (R)0};
-            zfold_bw<R>(seq, 0, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], one, one);
-            zfold_bw<R>(seq + NN, 1, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twQl[fr], twQl[fm]);
-            zfold_bw<R>(seq + 2 * NN, 2, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twq2<R>(twQl, NN, 2 * fr),
+            zfold_bw<R, LS>(seq, 0, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], one, one);
+            zfold_bw<R, LS>(seq + LPZ, 1, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twQl[fr], twQl[fm]);
+            zfold_bw<R, LS>(seq + 2 * LPZ, 2, fr, NN, pf[f][0], pf[f][1], pf[f][2], pf[f][3], twq2<R>(twQl, NN, 2 * fr),
                         twq2<R>(twQl, NN, 2 * fm));
         }
         if constexpr (BN + 1 < NBI) prefetch(BN + 1);
@@ -181,7 +187,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             constexpr int s1 = 2 * (BN * ZP + p);
 #pragma unroll
             for (int b = 0; b < 3; b++) {
-                const CT zv = buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
+                const CT zv = buf[((xl * ZP + p) * 3 + b) * LPZ + lslot<LS>(r)];
                 vb[b][s1] = zv.x;
                 if constexpr (s1 + 1 < KK) vb[b][s1 + 1] = zv.y;
             }
@@ -222,7 +228,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
                 R v1 = (R)0, v2 = (R)0;
                 if constexpr (r1 >= 0) v1 = FX::template row<r1>(vb[b]);
                 if constexpr (r2 >= 0) v2 = FX::template row<r2>(vb[b]);
-                buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)] = CT{v1, v2};
+                buf[((xl * ZP + p) * 3 + b) * LPZ + lslot<LS>(r)] = CT{v1, v2};
             }
         });
         __syncthreads();
@@ -234,8 +240,8 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             CT X[2] = {CT{(R)0, (R)0}, CT{(R)0, (R)0}};
 #pragma unroll
             for (int b = 0; b < 3; b++) {
-                const CT zn = buf[((xl * ZP + p) * 3 + b) * NN + pidx(r)];
-                const CT zm = buf[((xl * ZP + p) * 3 + b) * NN + pidx(rm)];
+                const CT zn = buf[((xl * ZP + p) * 3 + b) * LPZ + lslot<LS>(r)];
+                const CT zm = buf[((xl * ZP + p) * 3 + b) * LPZ + lslot<LS>(rm)];
                 const CT F1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
                 const CT F2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
                 if (b == 0) {
