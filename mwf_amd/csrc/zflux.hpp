@@ -150,12 +150,16 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     for (int i = tid; i < NN; i += T) twQl[i] = twQ[i];
     __syncthreads();
 
-    // Thread-derived indices are recomputed per batch from an opaque copy of
-    // the thread index, so that the compiler does not hoist the LDS addresses
-    // of every batch's passes into registers next to the point values.
+    // In FP64, thread-derived indices are recomputed per batch from an opaque
+    // copy of the thread index, so that the compiler does not hoist the LDS
+    // addresses of every batch's passes into registers next to the point
+    // values (hoisted, FP64 k_zc is 2 % slower at M4 1024^2).  The FP32
+    // kernels have the registers to hold them (at most 119 VGPRs, no spills):
+    // hoisted, they issue 9 % (NN = 512) to 22 % (NN = 1024) fewer VALU
+    // instructions.
     auto opaque_tid = [] {
         int t = threadIdx.x;
-        asm volatile("" : "+v"(t));
+        if constexpr (sizeof(R) == 8) asm volatile("" : "+v"(t));
         return t;
     };
 
