@@ -184,6 +184,7 @@ def config_m6_2048(mw, steps, err_steps, warmup=3):
             "speedup_mixed": mix["steps_per_s"] / f64["steps_per_s"], "steps": steps,
             "fp64_kernels_ms": {n: round(k["avg_ms"], 4) for n, k in f64["kernels"].items()},
             "mixed_kernels_ms": {n: round(k["avg_ms"], 4) for n, k in mix["kernels"].items()},
+            "fp64_plan": f64["plan"], "mixed_plan": mix["plan"],
             "mixed_vs_fp64": {"steps": err_steps, "energy_rel_err_per_step_max": float(e_err.max()),
                               "energy_rel_err_at_steps": {str(k): float(e_err[k - 1]) for k in
                                                           (1, 10, err_steps // 2, err_steps) if k <= err_steps},
@@ -424,6 +425,7 @@ def main():
     if mixed:
         line["mixed"] = {"steps_per_s": mixed["steps_per_s"],
                          "speedup_vs_fp64": mixed["steps_per_s"] / fp64["steps_per_s"],
+                         "plan": mixed["plan"],
                          "kernels_ms": {n: round(k["avg_ms"], 5) for n, k in mixed["kernels"].items()},
                          "energy_rel_err_vs_fp64": abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"]),
                          "finite": mixed["finite"]}

@@ -324,8 +324,11 @@ struct SizeClass {
         // every candidate runs 2 untimed steps (code load, first-step path),
         // then NR rounds, interleaved across candidates, of one untimed and NS
         // timed steps from a fresh zero state; a candidate's time is its best
-        // round, which keeps a single slow round from deciding the choice
-        constexpr int NR = 3, NS = 5;
+        // round, which keeps a single slow round from deciding the choice.
+        // Candidates a few percent apart (M4 1024^2 mixed: the one- and two-
+        // buffer z kernels) need this many steps to be ordered reliably; it
+        // costs about 0.2 s at M4 1024^2 and 1.5 s at M6 2048^2.
+        constexpr int NR = 5, NS = 10;
         std::vector<float> tbest(cand.size(), -1.0f);
         int rc = 0;
         for (size_t i = 0; i < cand.size() && !rc; i++) {
