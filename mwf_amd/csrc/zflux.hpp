@@ -41,14 +41,35 @@ template <typename R, int LOGNN, bool PP> constexpr int zc_pw() { return zc_wave
 // slower at M4 1024^2)
 template <typename R, int LOGNN, bool PP> constexpr int zc_ls() { return zc_wave<R, LOGNN, PP>() && sizeof(R) == 4 ? 4 : 0; }
 
+// Sequence-granular forward batches: with one x-row per block, lane-group
+// transforms and two buffers, a forward batch takes the next T/G (8)
+// sequences in the order (pair q, phase b) -> 3 q + b, so that every lane
+// group transforms; a pair whose three phases straddle two batches keeps its
+// partial sums in registers.  Pair-granular batches of ZP pairs (3 ZP = 6
+// sequences) leave two of the eight groups idle.  FP32 only: FP64's
+// transforms are bound by LDS bandwidth (16-byte values), and eight at once
+// made FP64 k_zc 2 % slower at M4 1024^2 despite one batch in nine fewer
+// (FP32 1 % faster).
+template <typename R, int LOGNN, bool PP> constexpr bool zc_sg()
+{
+    return sizeof(R) == 4 && PP && zc_wave<R, LOGNN, PP>() && zc_threads<LOGNN>() == (1 << LOGNN);
+}
+// sequences per buffer
+template <typename R, int LOGNN, int ZP, bool PP> constexpr int zc_bseq()
+{
+    constexpr int NN = 1 << LOGNN, T = zc_threads<LOGNN>(), SEQ = T / NN * ZP * 3;
+    constexpr int SPB = zc_sg<R, LOGNN, PP>() ? T / group_lanes<NN>() : SEQ;
+    return SEQ > SPB ? SEQ : SPB;
+}
+
 // transform buffer(s), then the pass twiddle table of the NN-point
 // transforms and the first NN entries of exp(2 pi i j/Q) (twq2 extends them)
 template <typename R, int LOGNN, int ZP, bool PP> constexpr size_t zc_lds_bytes()
 {
-    constexpr int NN = 1 << LOGNN, XB = zc_threads<LOGNN>() / NN;
+    constexpr int NN = 1 << LOGNN;
     constexpr int LPZ = lspan<NN, zc_ls<R, LOGNN, PP>()>();
-    return sizeof(cplx_t<R>) *
-           ((PP ? 2 : 1) * (size_t)XB * ZP * 3 * LPZ + ptw_size<NN, 1, zc_pw<R, LOGNN, PP>()>() + (size_t)NN);
+    return sizeof(cplx_t<R>) * ((PP ? 2 : 1) * (size_t)zc_bseq<R, LOGNN, ZP, PP>() * LPZ +
+                                ptw_size<NN, 1, zc_pw<R, LOGNN, PP>()>() + (size_t)NN);
 }
 
 template <typename R, int K0, int LOGNN, int ZP, int WAVES, bool PP>
@@ -77,10 +98,12 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     static_assert(FT * T == XB * ZP * H, "fold tasks must tile the block");
     extern __shared__ unsigned char smem_raw[];
     constexpr int LS = zc_ls<R, LOGNN, PP>(), LPZ = lspan<NN, LS>(); // sequence layout and span
-    CT* bufA = reinterpret_cast<CT*>(smem_raw);       // [SEQ][LPZ]: the sequences of a batch
-    CT* bufB = bufA + (PP ? SEQ * LPZ : 0);           // [SEQ][LPZ]: those of the next batch (PP)
+    constexpr bool SG = zc_sg<R, LOGNN, PP>();
+    constexpr int BSEQ = zc_bseq<R, LOGNN, ZP, PP>(); // sequences per buffer
+    CT* bufA = reinterpret_cast<CT*>(smem_raw);       // [BSEQ][LPZ]: the sequences of a batch
+    CT* bufB = bufA + (PP ? BSEQ * LPZ : 0);          // [BSEQ][LPZ]: those of the next batch (PP)
     constexpr int PW = zc_pw<R, LOGNN, PP>();
-    CT* twNl = bufB + SEQ * LPZ;                      // [ptw_size<NN, 1, PW>()] pass twiddles
+    CT* twNl = bufB + BSEQ * LPZ;                     // [ptw_size<NN, 1, PW>()] pass twiddles
     // buffer of batch k (inverse batches first, then forward ones)
     auto buf_of = [&](int k) { return (PP && (k & 1)) ? bufB : bufA; };
     // transforms of the first S sequences of b, in place, natural order in
@@ -105,7 +128,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         }
         __syncthreads();
     };
-    static_assert(SEQ * group_lanes<NN>() <= T, "one lane group per sequence");
+    static_assert(!WAVE || BSEQ * group_lanes<NN>() <= T, "one lane group per sequence");
     CT* twQl = twNl + ptw_size<NN, 1, PW>();          // [NN]: w^k, k < NN
 
     const int tid = threadIdx.x;
@@ -205,6 +228,79 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     const R gn = (R)(g.gamma * r);       // i gamma n at the thread's output n = r
     const R beta = (R)g.beta;
     CT acc[2 * KK];
+    // line assembly of the two rows q1, q1 + 1 of a forward pair from their
+    // transforms X: each row's terms go into the accumulators of its lines,
+    // and a line is stored after its last row
+    auto assemble = [&](auto q1c, const CT* X, CT* nlr, size_t nls) {
+        constexpr int q1 = decltype(q1c)::value;
+        static_for<0, 2>([&](auto hi) {
+            constexpr int q = q1 + decltype(hi)::value;
+            constexpr int row = FX::order(q);
+            if constexpr (row >= 0) {
+                const CT xs = cscale(X[decltype(hi)::value], scale);
+                static_for<0, FX::nuse(row)>([&](auto ui) {
+                    constexpr int u = decltype(ui)::value;
+                    constexpr int L = FX::use_line(row, u), kind = FX::use_kind(row, u);
+                    constexpr int sg = FX::use_sign(row, u);
+                    CT term = xs;
+                    if constexpr (kind == 1) term = cscale(xs, (R)sg * beta);
+                    else if constexpr (kind == 2) term = CT{-(xs.y * gn), xs.x * gn};
+                    if constexpr (FX::first(L) == q) acc[L] = term;
+                    else acc[L] = cadd(acc[L], term);
+                    if constexpr (FX::last(L) == q) {
+                        if (valid) st_piece(nlr + L * nls, acc[L], wide);
+                    }
+                });
+            }
+        });
+    };
+    if constexpr (SG) {
+        constexpr int SPB = T / group_lanes<NN>();   // sequences per forward batch
+        constexpr int NSF = 3 * NPO;                 // forward sequences: pair q, phase b at 3 q + b
+        constexpr int NBF = (NSF + SPB - 1) / SPB;
+        CT X[2];                                      // transforms of the current pair's two rows
+        static_for<0, NBF>([&](auto ri) {
+            constexpr int RD = decltype(ri)::value;
+            constexpr int J0 = RD * SPB, J1 = J0 + SPB < NSF ? J0 + SPB : NSF;
+            const int tid = opaque_tid();
+            const int r = tid & (NN - 1); // one x-row per block
+            const int rm = (NN - r) & (NN - 1);
+            CT* buf = buf_of(NBI + RD);
+            CT* nlr = NL + ((((size_t)(r >> g.lgNT) * (2 * KK) * g.P) + x0) << g.lgNT) + (r & (g.NTx - 1));
+            const size_t nls = (size_t)g.P << g.lgNT;
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int sl = 0; sl < KK; sl++) asm volatile("" : "+v"(vb[b][sl]));
+            static_for<J0, J1>([&](auto ji) {
+                constexpr int j = decltype(ji)::value, q = j / 3, b = j % 3;
+                constexpr int r1 = FX::order(2 * q), r2 = FX::order(2 * q + 1);
+                R v1 = (R)0, v2 = (R)0;
+                if constexpr (r1 >= 0) v1 = FX::template row<r1>(vb[b]);
+                if constexpr (r2 >= 0) v2 = FX::template row<r2>(vb[b]);
+                buf[(j - J0) * LPZ + lslot<LS>(r)] = CT{v1, v2};
+            });
+            __syncthreads();
+            transform(std::integral_constant<int, -1>{}, J1 - J0, buf, tid);
+            static_for<J0, J1>([&](auto ji) {
+                constexpr int j = decltype(ji)::value, q = j / 3, b = j % 3;
+                const CT zn = buf[(j - J0) * LPZ + lslot<LS>(r)];
+                const CT zm = buf[(j - J0) * LPZ + lslot<LS>(rm)];
+                const CT F1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
+                const CT F2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
+                if constexpr (b == 0) {
+                    X[0] = F1;
+                    X[1] = F2;
+                } else {
+                    const CT w = cconj(twq2<R>(twQl, NN, r * b));
+                    X[0] = cadd(X[0], cmul(F1, w));
+                    X[1] = cadd(X[1], cmul(F2, w));
+                }
+                if constexpr (b == 2) assemble(std::integral_constant<int, 2 * q>{}, X, nlr, nls);
+            });
+        });
+        return;
+    }
     static_for<0, NBO>([&](auto ri) {
         constexpr int RD = decltype(ri)::value;
         constexpr int NV = NPO - RD * ZP < ZP ? NPO - RD * ZP : ZP; // valid output pairs
@@ -257,26 +353,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
                     X[1] = cadd(X[1], cmul(F2, w));
                 }
             }
-            static_for<0, 2>([&](auto hi) {
-                constexpr int q = q1 + decltype(hi)::value;
-                constexpr int row = FX::order(q);
-                if constexpr (row >= 0) {
-                    const CT xs = cscale(X[decltype(hi)::value], scale);
-                    static_for<0, FX::nuse(row)>([&](auto ui) {
-                        constexpr int u = decltype(ui)::value;
-                        constexpr int L = FX::use_line(row, u), kind = FX::use_kind(row, u);
-                        constexpr int sg = FX::use_sign(row, u);
-                        CT term = xs;
-                        if constexpr (kind == 1) term = cscale(xs, (R)sg * beta);
-                        else if constexpr (kind == 2) term = CT{-(xs.y * gn), xs.x * gn};
-                        if constexpr (FX::first(L) == q) acc[L] = term;
-                        else acc[L] = cadd(acc[L], term);
-                        if constexpr (FX::last(L) == q) {
-                            if (valid) st_piece(nlr + L * nls, acc[L], wide);
-                        }
-                    });
-                }
-            });
+            assemble(std::integral_constant<int, q1>{}, X, nlr, nls);
         });
         if constexpr (!PP) __syncthreads();
     });
