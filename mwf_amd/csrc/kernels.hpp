@@ -1340,8 +1340,8 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
 // group of G = min(64, MM/8) lanes owns one of the 3*NT sequences (t, b), 8
 // points per lane (16 at MM = 1024), and transforms it with group_fft (no
 // block barrier inside a transform); items (t, r), up to XTILE/XS per thread,
-// hold psi_k / phi_k at m = r and m = r - MM.  768 threads for MM <= 512,
-// 384 at MM = 1024.
+// hold psi_k / phi_k at m = r and m = r - MM.  768 threads at every MM (at
+// MM = 1024, twice the 384 lanes of its six sequences).
 // ---------------------------------------------------------------------------
 // Threads per block: one lane group per sequence, 3 XTILE/MM sequences; at MM
 // = 1024 (6 sequences) twice that, so that a thread holds 3 items instead of
