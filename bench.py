@@ -49,23 +49,21 @@ def step_bytes(mw, p):
     return 2 * 16 * mpt + 2 * si * mpt + 2 * 3 * d["KK"] * d["P"] * p.nn * si
 
 
-def run_gpu(mw, precision, steps, warmup, ic, workload=WORKLOAD):
+def run_gpu(mw, precision, steps, warmup, ic, workload=WORKLOAD, event_steps=200):
     p = mw.make_params(precision=precision, **workload)
     v = mw.Velocity(p)
     plan = v.plan_info()
     v.set_state(*ic)
-    v.vel_TS(warmup)
-    v.sync()
-    # the timed steps run as the product runs, with no events between kernels
-    t0 = time.perf_counter()
-    v.vel_TS(steps)
-    v.sync()
-    t1 = time.perf_counter()
-    # then the same number of steps with HIP events around every kernel on the
-    # library's stream, for the per-kernel durations (the events themselves
-    # cost a few microseconds per launch, so they stay out of the timed steps)
+    # the first step after a state upload also rebuilds U (inverse-only x pass)
+    v.vel_TS(1)
+    # per-kernel durations first: max(steps, event_steps) steps with HIP events
+    # around every kernel on the library's stream (the events cost a few
+    # microseconds per launch, so they stay out of the timed steps).  Run
+    # before the warm-up, this pass also brings the device to the clocks of a
+    # running simulation: after the idle gap of the state upload, a handful of
+    # warm-up steps leave the first milliseconds of a short timed run slow.
     v.profile(True)
-    v.vel_TS(steps)
+    v.vel_TS(max(steps, event_steps))
     v.sync()
     kern = {}
     for kid, name in mw.KERNEL_NAMES.items():
@@ -74,6 +72,13 @@ def run_gpu(mw, precision, steps, warmup, ic, workload=WORKLOAD):
             kern[name] = dict(kid=kid, total_ms=ms, count=cnt, avg_ms=ms / cnt,
                               bytes=v.kernel_bytes(kid))
     v.profile(False)
+    v.vel_TS(warmup)
+    v.sync()
+    # the timed steps run as the product runs, with no events between kernels
+    t0 = time.perf_counter()
+    v.vel_TS(steps)
+    v.sync()
+    t1 = time.perf_counter()
     re, im = v.get_state()
     finite = bool(np.isfinite(re).all() and np.isfinite(im).all())
     e = v.vel_energy()
@@ -321,7 +326,7 @@ def profile_case(name, steps, warmup, no_mixed, tune):
     state = mw.ic_gaussian(p, ic["seed"], ic["scl"], ic["k0"])
     out = {"profile_case": name}
     for prec in [mw.MWF_FP64] + ([] if no_mixed else [mw.MWF_MIXED]):
-        r = run_gpu(mw, prec, steps, warmup, state, wl)
+        r = run_gpu(mw, prec, steps, warmup, state, wl, event_steps=0)
         out["fp64" if prec == mw.MWF_FP64 else "mixed"] = {
             "steps_per_s": r["steps_per_s"], "plan": r["plan"], "finite": r["finite"],
             "kernels_ms": {n: round(k["avg_ms"], 5) for n, k in r["kernels"].items()}}
