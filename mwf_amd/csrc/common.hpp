@@ -70,6 +70,8 @@ struct Geo {
     int NN, P, Q;      // z modes, padded grid P = 3MM (x), Q = 3NN (z)
     int NTx;           // z-modes per x-kernel block (power of two)
     int lgNT;          // log2(NTx)
+    int NTs;           // z-modes per NL piece (a multiple of NTx, power of two)
+    int lgNS;          // log2(NTs)
     int XB;            // x-rows per z-kernel block
     int UL, NLL;       // lines per x-row of U and NL: 2 KK and KK (advective form), KK and 2 KK (flux form)
     double alpha, gamma, re, dt, drag, vdrag, hypo, cth, sth, beta;
@@ -101,23 +103,29 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
 
 // U (the x-transformed lines, UL per x row) is stored U[x][line][n]: the z
 // kernel reads whole rows.  NL (the z-forward products, NLL lines) is stored
-// in panels NL[tile][line][x][t], n = tile*NTx + t: the x kernels read whole
-// panels.
-// The writers of both store NTx- or NN-long pieces; partial-line writes cost
-// no extra HBM reads, partial-line reads would.
+// in panels NL[tile][line][x][s], n = tile*NTs + s: a z-kernel thread writes
+// NTs-long pieces, and an x-kernel block, which takes NTx <= NTs columns,
+// reads its NTx-wide stripe of a panel.  With NTs = NTx the stripe is the
+// whole panel; FP64 uses NTs = 2 NTx at MM >= 512 (128-byte pieces at 512,
+// whole L2 lines for the writers; the two blocks that read the halves of each
+// line run side by side on one XCD, so the line is fetched once).
+// U's writers store NTx- or NN-long pieces; partial-line writes cost no extra
+// HBM reads, partial-line reads would.
 __device__ __forceinline__ size_t uidx(const Geo& g, int x, int line, int n)
 {
     return ((size_t)x * g.UL + line) * g.NN + n;
 }
-// NL panel (tile of n0, slot): P * NTx contiguous values, element x * NTx + t.
+// NL stripe of an x-kernel block (columns n0 .. n0 + NTx - 1, slot): element
+// i = x * NTx + t at offset pan_at(g, i) from the returned base.
 template <typename CT> __device__ __forceinline__ const CT* nl_panel(const Geo& g, const CT* NL, int n0, int slot)
 {
-    return NL + ((size_t)(n0 >> g.lgNT) * g.NLL + slot) * g.P * g.NTx;
+    return NL + ((size_t)(n0 >> g.lgNS) * g.NLL + slot) * g.P * g.NTs + (n0 & (g.NTs - 1));
 }
+__device__ __forceinline__ int pan_at(const Geo& g, int i) { return ((i >> g.lgNT) << g.lgNS) + (i & (g.NTx - 1)); }
 __device__ __forceinline__ size_t nidx(const Geo& g, int x, int slot, int n)
 {
-    const int tile = n >> g.lgNT, t = n & (g.NTx - 1);
-    return (((size_t)tile * g.NLL + slot) * g.P + x) * g.NTx + t;
+    const int tile = n >> g.lgNS, t = n & (g.NTs - 1);
+    return (((size_t)tile * g.NLL + slot) * g.P + x) * g.NTs + t;
 }
 
 // wall-normal derivative table ad_k1, ad_k2 (program/variables.f90:528-531)

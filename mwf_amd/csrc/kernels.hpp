@@ -477,7 +477,7 @@ __global__ __launch_bounds__(ZT) void k_z(Batch bt, const cplx_t<R>* __restrict_
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * XB;
     const int x = x0 + xl;
     const bool valid = x < g.P;
-    const bool wide = g.NTx * (int)sizeof(CT) >= 64;
+    const bool wide = g.NTs * (int)sizeof(CT) >= 64;
 
     // fold task f of this thread: tau = tid + ZT*f -> (row fx, pair fp, index fr)
     CT pf[FT][4];
@@ -676,7 +676,7 @@ __global__ __launch_bounds__(Z2T) void k_z2(Batch bt, const cplx_t<R>* __restric
     CT* twQl = twNl + ptw_size<NN>();          // [NN]: w^k, k < NN
 
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * XB;
-    const bool wide = g.NTx * (int)sizeof(CT) >= 64;
+    const bool wide = g.NTs * (int)sizeof(CT) >= 64;
     // Thread-derived indices are recomputed per batch from an opaque copy of
     // the thread index, so that the compiler does not hoist every batch's
     // addresses into registers next to the point state.
@@ -1228,9 +1228,10 @@ __global__ __launch_bounds__(XT) void k_xfwd(Batch bt, const cplx_t<R>* __restri
             if (it >= 3 * nitems) continue;
             const int x = it >> g.lgNT, t = it & (NT - 1);
             const int a = x / 3, b = x - 3 * a;
-            CT v = pan[it];
+            const int po = pan_at(g, it);
+            CT v = pan[po];
             if (bt.nl_part) { // the z pass's three fold-phase partials: ((P0 + P1) + P2) / (PQ)
-                const CT v1 = pan[it + bt.nl_part], v2 = pan[it + 2 * bt.nl_part];
+                const CT v1 = pan[po + bt.nl_part], v2 = pan[po + 2 * bt.nl_part];
                 v = cscale(cadd(cadd(v, v1), v2), (R)g.scale);
             }
             buf[((size_t)t * 3 + b) * LP + pidx(a)] = v;
@@ -1471,7 +1472,9 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
             return nl_panel(g, NL, n0, slot + part * g.KK);
         };
-        const int nch = nrows * (int)sizeof(CT) / 16; // 16-byte chunks of one panel
+        // 16-byte chunks of one panel (the mixed form keeps NTs = NTx, so a
+        // block's stripe is a contiguous panel)
+        const int nch = nrows * (int)sizeof(CT) / 16;
         if constexpr (ASYNC) panel_to_lds<XS>(panel_of(0), stage, nch, tid);
 #pragma unroll 1
         for (int li = 0; li < NLI; li++) {
@@ -1521,9 +1524,13 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                     int t0, xq, pa;
                     CT* rb;
                     tile_item(tidv, t0, xq, rb, pa);
+                    // item q is element tidv + XS q of the stripe: XS/NTx
+                    // x-rows further
+                    const CT* pb = pan + pan_at(g, tidv);
+                    const int pstep = (XS >> LOGNTC) << g.lgNS;
                     static_for<0, LPT>([&](auto qi) {
                         constexpr int q = decltype(qi)::value;
-                        swz<q * DA>(rb, pa) = nl_at(pan + tidv + XS * q);
+                        swz<q * DA>(rb, pa) = nl_at(pb + q * pstep);
                     });
                 } else {
 #pragma unroll
@@ -1532,7 +1539,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                         if (i >= nrows) continue;
                         const int x = i >> g.lgNT, t = i & (NT - 1);
                         const int a = x / 3, b = x - 3 * a;
-                        reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = nl_at(pan + i);
+                        reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = nl_at(pan + pan_at(g, i));
                     }
                 }
                 __syncthreads();

@@ -104,6 +104,14 @@ static Geo geo_of(const mwf_params* p)
     g.Q = 3 * p->nn;
     g.NTx = std::min(p->nn, XTILE / p->mm);
     g.lgNT = ilog2(g.NTx);
+    // NL pieces: in FP64, two x blocks' columns where a block's are under
+    // 128 bytes (MM = 512: 128-byte pieces, k_zc 16 % faster at M4 1024^2
+    // with k_xstep unchanged; MM = 1024: 64-byte pieces, M6 2048^2 182 -> 193
+    // steps/s, where 128-byte pieces cost the x kernel more than they save);
+    // NTx in the mixed form, whose x kernels stage whole panels with
+    // global_load_lds
+    g.NTs = p->precision == MWF_FP64 && g.NTx < 8 ? std::min(p->nn, 2 * g.NTx) : g.NTx;
+    g.lgNS = ilog2(g.NTs);
     g.XB = ZT / p->nn;
     g.UL = 2 * g.KK;
     g.NLL = g.KK;

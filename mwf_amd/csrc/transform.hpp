@@ -164,7 +164,7 @@ __global__ __launch_bounds__(XT) void k_sxfwd(Geo g, const cplx_t<R>* __restrict
         if (it >= 3 * nitems) continue;
         const int x = it >> g.lgNT, t = it & (NT - 1);
         const int a = x / 3, b = x - 3 * a;
-        buf[((size_t)t * 3 + b) * LP + pidx(a)] = pan[it];
+        buf[((size_t)t * 3 + b) * LP + pidx(a)] = pan[pan_at(g, it)];
     }
     __syncthreads();
     ip_fft<CT, LOGMM, -1, XT, 3 * (XTILE >> LOGMM)>(buf, 3 * NT, twMM);

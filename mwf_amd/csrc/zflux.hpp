@@ -136,7 +136,7 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     const int x0 = xcd_tile(blockIdx.x, gridDim.x) * XB;
     const int x = x0 + xl;
     const bool valid = x < g.P;
-    const bool wide = g.NTx * (int)sizeof(CT) >= 64;
+    const bool wide = g.NTs * (int)sizeof(CT) >= 64;
     // with one x-row per block the sequences of the valid pairs of a batch
     // come first, so a part-padding batch transforms only those
     auto nseq = [](int nv) { return XB == 1 ? 3 * nv : SEQ; };
@@ -266,8 +266,8 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
             const int r = tid & (NN - 1); // one x-row per block
             const int rm = (NN - r) & (NN - 1);
             CT* buf = buf_of(NBI + RD);
-            CT* nlr = NL + ((((size_t)(r >> g.lgNT) * (2 * KK) * g.P) + x0) << g.lgNT) + (r & (g.NTx - 1));
-            const size_t nls = (size_t)g.P << g.lgNT;
+            CT* nlr = NL + ((((size_t)(r >> g.lgNS) * (2 * KK) * g.P) + x0) << g.lgNS) + (r & (g.NTs - 1));
+            const size_t nls = (size_t)g.P << g.lgNS;
 #pragma unroll
             for (int b = 0; b < 3; b++)
 #pragma unroll
@@ -310,8 +310,8 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         // this thread's element of NL line L is nlr[L * nls] (nidx with the
         // flux form's 2 KK lines): one base per batch instead of the full
         // index per store
-        CT* nlr = NL + ((((size_t)(r >> g.lgNT) * (2 * KK) * g.P) + (x0 + xl)) << g.lgNT) + (r & (g.NTx - 1));
-        const size_t nls = (size_t)g.P << g.lgNT;
+        CT* nlr = NL + ((((size_t)(r >> g.lgNS) * (2 * KK) * g.P) + (x0 + xl)) << g.lgNS) + (r & (g.NTs - 1));
+        const size_t nls = (size_t)g.P << g.lgNS;
         // the base values pass through an empty asm per batch: products shared
         // by rows of different batches are formed again in each batch instead
         // of being computed once and held in registers across all of them
