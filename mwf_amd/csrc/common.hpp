@@ -106,9 +106,10 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
 // in panels NL[tile][line][x][s], n = tile*NTs + s: a z-kernel thread writes
 // NTs-long pieces, and an x-kernel block, which takes NTx <= NTs columns,
 // reads its NTx-wide stripe of a panel.  With NTs = NTx the stripe is the
-// whole panel; FP64 uses NTs = 2 NTx at MM >= 512 (128-byte pieces at 512,
-// whole L2 lines for the writers; the two blocks that read the halves of each
-// line run side by side on one XCD, so the line is fetched once).
+// whole panel.  Where NTx values are under 128 bytes (FP64 at MM >= 512, FP32
+// at MM >= 256), NTs = 2 NTx: the writers store twice as long runs (whole L2
+// lines at FP64 MM = 512), and the two blocks that read the halves of each
+// piece run side by side on one XCD, so the lines are fetched once.
 // U's writers store NTx- or NN-long pieces; partial-line writes cost no extra
 // HBM reads, partial-line reads would.
 __device__ __forceinline__ size_t uidx(const Geo& g, int x, int line, int n)

@@ -1369,19 +1369,20 @@ template <typename R> __device__ __forceinline__ int xsw(int t, int b)
 }
 template <typename R> constexpr size_t xstep_stage_bytes() { return xstep_async<R>() ? 3 * (size_t)XTILE * sizeof(cplx_t<R>) : 0; }
 
-// Wave-cooperative copy of a contiguous panel of nch 16-byte chunks from
-// global memory into LDS, asynchronously: lane l of a wave moves chunk c0 + l
-// to dst + 16 (c0 + l), c0 a multiple of 64 taken by the wave.  nch is a
-// multiple of 64.  The data is in LDS once the wave's vector memory count
-// has drained and a barrier has passed.
+// Wave-cooperative copy of a block's NL stripe (nch 16-byte chunks, chunk c
+// holding stripe elements c E .. c E + E - 1, E = 16 / sizeof(CT), which lie
+// in one x-row's piece) from global memory into LDS, asynchronously: lane l of
+// a wave moves chunk c0 + l to dst + 16 (c0 + l), c0 a multiple of 64 taken by
+// the wave.  nch is a multiple of 64.  The data is in LDS once the wave's
+// vector memory count has drained and a barrier has passed.
 template <int T, typename CT>
-__device__ __forceinline__ void panel_to_lds(const CT* __restrict__ src, CT* dst, int nch, int tid)
+__device__ __forceinline__ void panel_to_lds(const Geo& g, const CT* __restrict__ src, CT* dst, int nch, int tid)
 {
+    constexpr int E = 16 / (int)sizeof(CT);
     const int wave = tid >> 6, lane = tid & 63;
-    const char* g = reinterpret_cast<const char*>(src);
     char* l = reinterpret_cast<char*>(dst);
     for (int c0 = wave * 64; c0 < nch; c0 += T)
-        __builtin_amdgcn_global_load_lds(g + (size_t)(c0 + lane) * 16,
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const char*>(src + pan_at(g, (c0 + lane) * E)),
                                          (__attribute__((address_space(3))) void*)(l + (size_t)c0 * 16), 16, 0, 0);
 }
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
@@ -1472,10 +1473,8 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             const int slot = comp == 0 ? k : (comp == 1 ? g.K0 - 1 + k : 2 * g.K0 - 1 + k);
             return nl_panel(g, NL, n0, slot + part * g.KK);
         };
-        // 16-byte chunks of one panel (the mixed form keeps NTs = NTx, so a
-        // block's stripe is a contiguous panel)
-        const int nch = nrows * (int)sizeof(CT) / 16;
-        if constexpr (ASYNC) panel_to_lds<XS>(panel_of(0), stage, nch, tid);
+        const int nch = nrows * (int)sizeof(CT) / 16; // 16-byte chunks of one stripe
+        if constexpr (ASYNC) panel_to_lds<XS>(g, panel_of(0), stage, nch, tid);
 #pragma unroll 1
         for (int li = 0; li < NLI; li++) {
             const int comp = FLUX ? li >> 1 : li, part = FLUX ? li & 1 : 0;
@@ -1509,7 +1508,7 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
                 __syncthreads();
                 int ln = li + 1;
                 while (ln < NLI && !line_on(ln)) ln++;
-                if (ln < NLI) panel_to_lds<XS>(panel_of(ln), stage, nch, tid);
+                if (ln < NLI) panel_to_lds<XS>(g, panel_of(ln), stage, nch, tid);
             } else {
                 const CT* pan = panel_of(li);
                 // NL element, or the sum ((P0 + P1) + P2) / (PQ) of the three

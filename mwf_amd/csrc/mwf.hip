@@ -104,13 +104,15 @@ static Geo geo_of(const mwf_params* p)
     g.Q = 3 * p->nn;
     g.NTx = std::min(p->nn, XTILE / p->mm);
     g.lgNT = ilog2(g.NTx);
-    // NL pieces: in FP64, two x blocks' columns where a block's are under
-    // 128 bytes (MM = 512: 128-byte pieces, k_zc 16 % faster at M4 1024^2
-    // with k_xstep unchanged; MM = 1024: 64-byte pieces, M6 2048^2 182 -> 193
-    // steps/s, where 128-byte pieces cost the x kernel more than they save);
-    // NTx in the mixed form, whose x kernels stage whole panels with
-    // global_load_lds
-    g.NTs = p->precision == MWF_FP64 && g.NTx < 8 ? std::min(p->nn, 2 * g.NTx) : g.NTx;
+    // NL pieces: two x blocks' columns where a block's are under 128 bytes
+    // (FP64 at MM >= 512, FP32 at MM >= 256), so that the z kernels write
+    // longer runs and the two blocks that read them run side by side on one
+    // XCD.  FP64 M4 1024^2 (128-byte pieces): k_zc 0.196 -> 0.163 ms, k_xstep
+    // unchanged; FP64 M6 2048^2 (64-byte): 182 -> 193 steps/s; mixed 4770 ->
+    // 4868 (M4) and 568 -> 586 (M6).  Four blocks' columns per piece cost the
+    // x kernels more than they save.
+    const int cb = p->precision == MWF_FP64 ? 16 : 8; // bytes per complex value in NL
+    g.NTs = g.NTx * cb < 128 ? std::min(p->nn, 2 * g.NTx) : g.NTx;
     g.lgNS = ilog2(g.NTs);
     g.XB = ZT / p->nn;
     g.UL = 2 * g.KK;
