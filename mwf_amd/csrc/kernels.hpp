@@ -1412,6 +1412,12 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
     static_assert(ptw_size<MM, 1, 7>() % 2 == 0, "twPl must stay 16-byte aligned");
     double2* twPl = reinterpret_cast<double2*>(twMl + ptw_size<MM, 1, 7>()); // exp(2 pi i j/P), j <= 2 MM
     constexpr bool ASYNC = xstep_async<R>() && !INV_ONLY;
+    // FP32 at MM <= 512: the forward transforms' first pass reads a whole
+    // tile's sequences straight from the staged panel (element a of sequence
+    // (t, b) at panel row 3a + b, column t), with no copy into the transform
+    // layout and no barrier between (k_xstep 0.130 -> 0.123 ms at M4 1024^2);
+    // at MM = 1024 that form spills 32 VGPRs and is 30 % slower
+    constexpr bool SREAD = ASYNC && LOGMM < 10;
     CT* stage = reinterpret_cast<CT*>(twPl + 2 * MM + 1);               // [3 XTILE] next NL panel
 
     const Geo g = bt.geo[blockIdx.z];
@@ -1483,32 +1489,35 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             int tidv = tid;
             asm volatile("" : "+v"(tidv));
             if constexpr (ASYNC) {
-                // this line's panel has landed in the staging area: lay it out
-                // for the transforms, then start the next line's panel
+                // this line's panel has landed in the staging area.  A whole
+                // tile's transforms read it there (MM <= 512); otherwise it is
+                // laid out for them first, and the next line's panel starts then
                 wait_vmem();
                 __syncthreads();
-                if (whole) {
-                    int t0, xq, pa;
-                    CT* rb;
-                    tile_item(tidv, t0, xq, rb, pa);
-                    static_for<0, LPT>([&](auto qi) {
-                        constexpr int q = decltype(qi)::value;
-                        swz<q * DA>(rb, pa) = stage[tidv + XS * q];
-                    });
-                } else {
+                if (!(SREAD && whole)) {
+                    if (whole) {
+                        int t0, xq, pa;
+                        CT* rb;
+                        tile_item(tidv, t0, xq, rb, pa);
+                        static_for<0, LPT>([&](auto qi) {
+                            constexpr int q = decltype(qi)::value;
+                            swz<q * DA>(rb, pa) = stage[tidv + XS * q];
+                        });
+                    } else {
 #pragma unroll
-                    for (int q = 0; q < LPT; q++) {
-                        const int i = tidv + XS * q;
-                        if (i >= nrows) continue;
-                        const int x = i >> g.lgNT, t = i & (NT - 1);
-                        const int a = x / 3, b = x - 3 * a;
-                        reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = stage[i];
+                        for (int q = 0; q < LPT; q++) {
+                            const int i = tidv + XS * q;
+                            if (i >= nrows) continue;
+                            const int x = i >> g.lgNT, t = i & (NT - 1);
+                            const int a = x / 3, b = x - 3 * a;
+                            reg[(t * 3 + b) * MM + (pidx(a) ^ xsw<R>(t, b))] = stage[i];
+                        }
                     }
+                    __syncthreads();
+                    int ln = li + 1;
+                    while (ln < NLI && !line_on(ln)) ln++;
+                    if (ln < NLI) panel_to_lds<XS>(g, panel_of(ln), stage, nch, tid);
                 }
-                __syncthreads();
-                int ln = li + 1;
-                while (ln < NLI && !line_on(ln)) ln++;
-                if (ln < NLI) panel_to_lds<XS>(g, panel_of(ln), stage, nch, tid);
             } else {
                 const CT* pan = panel_of(li);
                 // NL element, or the sum ((P0 + P1) + P2) / (PQ) of the three
@@ -1546,11 +1555,25 @@ __global__ __launch_bounds__(xs_of<LOGMM>()) void k_xstep(Batch bt, const cplx_t
             if (seq_on) {
                 CT v[PL];
                 const int pg = pidx(gl) ^ sxo;
-                static_for<0, PL>([&](auto qi) { v[qi] = swz<decltype(qi)::value * G>(myreg, pg); });
+                if (SREAD && whole) {
+                    // element a of sequence (t, b) = (s/3, s%3) is panel row
+                    // x = 3a + b, column t of the staged panel
+                    const CT* sb = stage + (3 * gl + s % 3) * NTC + s / 3;
+                    static_for<0, PL>([&](auto qi) { v[qi] = sb[3 * G * NTC * decltype(qi)::value]; });
+                } else {
+                    static_for<0, PL>([&](auto qi) { v[qi] = swz<decltype(qi)::value * G>(myreg, pg); });
+                }
                 group_fft<CT, LOGMM, -1, true, 7>(v, myreg, gl, twMl, sxo);
                 static_for<0, PL>([&](auto qi) { swz<decltype(qi)::value * G>(myreg, pg) = v[qi]; });
             }
             __syncthreads();
+            if constexpr (SREAD) {
+                if (whole) { // the staged panel has been read: start the next line's
+                    int ln = li + 1;
+                    while (ln < NLI && !line_on(ln)) ln++;
+                    if (ln < NLI) panel_to_lds<XS>(g, panel_of(ln), stage, nch, tid);
+                }
+            }
 #pragma unroll
             for (int q = 0; q < IPT; q++) {
                 const int it = tidv + XS * q;
