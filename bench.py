@@ -87,7 +87,7 @@ def run_gpu(mw, precision, steps, warmup, ic, workload=WORKLOAD, event_steps=200
                 energy=e, plan=plan)
 
 
-def ensemble_cfg4(mw, nsteps, seq_members=16, seq_steps=20, track_every=50):
+def ensemble_m4_128_x256(mw, nsteps, seq_members=16, seq_steps=20, track_every=50):
     """256 M4 members, 128x128 modes, Lx=Lz=16pi, Re ~ U[150,220], rms ~ U[0.05,0.4],
     seeds 1000..1255, dt 0.02, FP64: batched member-steps/s over nsteps (5000 by
     default), with every member's energy traced every track_every steps and its
@@ -207,7 +207,7 @@ def config_m4_64(mw, steps):
             "steps_per_s": r["steps_per_s"], "steps": steps, "finite": r["finite"], "energy": r["energy"]}
 
 
-def ensemble_cfg5(mw, nsteps, subset=32, track_every=50):
+def ensemble_uneven_x512(mw, nsteps, subset=32, track_every=50):
     """512 members, M4:M6 = 3:1, sizes {64,128,256,512}^2 modes, box 0.4 pi per
     mode, Re ~ U[150,300], seeds 5000..5511, dt 0.01, mixed form; energy error
     of a 32-member FP64 subset after the same steps."""
@@ -364,7 +364,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="rule-based kernel plans instead of timed ones")
     ap.add_argument("--plan", type=int, default=None,
                     help="force kernel plan bits (1 k_zb, 2 split x, 4 k_z2) instead of timing the plans")
-    ap.add_argument("--cfg5-steps", type=int, default=2000)
+    ap.add_argument("--uneven-steps", type=int, default=2000)
     ap.add_argument("--calib-copy", type=int, default=0, help="only run a MiB-sized device copy")
     ap.add_argument("--profile-case", choices=sorted(PROFILE_CASES), default=None,
                     help="only step one workload (FP64, then mixed) for --steps steps after --warmup, "
@@ -438,8 +438,8 @@ def main():
         line["configs"] = {"m4_64": config_m4_64(mw, 1000),
                            "m6_2048": config_m6_2048(mw, args.m6_steps, args.m6_err_steps)}
     if not args.no_ensemble:
-        line["ensemble"] = {"m4_128_x256": ensemble_cfg4(mw, args.ens_steps),
-                            "uneven_x512_mixed": ensemble_cfg5(mw, args.cfg5_steps)}
+        line["ensemble"] = {"m4_128_x256": ensemble_m4_128_x256(mw, args.ens_steps),
+                            "uneven_x512_mixed": ensemble_uneven_x512(mw, args.uneven_steps)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ic, args.cpu_steps)
     print(json.dumps(line), flush=True)

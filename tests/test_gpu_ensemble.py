@@ -83,8 +83,8 @@ def test_relaminarisation_tracking():
     assert np.all(e1[1:][np.arange(20) * 10 + 10 >= r1] <= 5e-7 * 1.5)
 
 
-def test_uneven_mixed_ensemble_cfg5_shapes():
-    """The uneven ensemble's shape: M4 and M6 members of
+def test_uneven_mixed_ensemble_512_member_shapes():
+    """The shape of the 512-member uneven ensemble: M4 and M6 members of
     64^2..512^2 modes in the mixed form, box 0.4 pi per mode, Re in [150, 300],
     dt 0.01, plus one FP64 member.  Every member equals its single-model run
     in its own precision bit for bit; the FP64 member matches the oracle; each
@@ -106,7 +106,7 @@ def test_uneven_mixed_ensemble_cfg5_shapes():
     ens.step(nsteps)
     e = ens.energies()
     for i, p in enumerate(ps):
-        print(f"[cfg5] member {i}: k0={p.k0} mm={p.mm} prec={p.precision}", flush=True)
+        print(f"[uneven] member {i}: k0={p.k0} mm={p.mm} prec={p.precision}", flush=True)
         single = mw.Velocity(p)
         single.set_state(*ics[i])
         single.vel_TS(nsteps)
@@ -130,9 +130,9 @@ def test_uneven_mixed_ensemble_cfg5_shapes():
     ens.close()
 
 
-def test_256_member_class_cfg4_shape():
-    """One size class of 256 M4 members of 128^2 modes (
-    L = 16 pi, Re in [150, 220], scl in [0.05, 0.4], seeds 1000..1255): after
+def test_256_member_class_m4_128():
+    """One size class of 256 M4 members of 128^2 modes (the 256-member
+    ensemble workload: L = 16 pi, Re in [150, 220], scl in [0.05, 0.4], seeds 1000..1255): after
     10 steps every member equals its single-model run bit for bit."""
     rng = np.random.default_rng(1000)
     res = rng.uniform(150.0, 220.0, 256)

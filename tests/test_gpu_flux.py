@@ -101,7 +101,7 @@ def test_flux_plan_mixed_form(tuning, k0, mm, nn, choice):
     v.close()
 
 
-def test_flux_cfg2_shape(tuning):
+def test_flux_m4_1024_shape(tuning):
     """The headline workload's shape (M4, 1024^2 modes, Lx = Lz = 128 pi,
     Re 180, dt 0.02, seed 2) through the flux plan, 20 steps against the
     oracle."""
