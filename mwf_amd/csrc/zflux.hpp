@@ -256,7 +256,14 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
     };
     if constexpr (SG) {
         constexpr int SPB = T / group_lanes<NN>();   // sequences per forward batch
-        constexpr int NSF = 3 * NPO;                 // forward sequences: pair q, phase b at 3 q + b
+        // forward sequences: pair q, phase b at 3 q + b.  With an odd number
+        // of rows the last pair holds one row, whose three phases take two
+        // sequences (phases 0 and 1 as the real and imaginary parts of the
+        // first, phase 2 in the second): M4's 21 rows fill 4 batches of 8
+        // instead of 5
+        constexpr bool LONE = NR % 2 == 1;
+        constexpr int QL = NPO - 1;                  // the last pair
+        constexpr int NSF = 3 * NPO - (LONE ? 1 : 0);
         constexpr int NBF = (NSF + SPB - 1) / SPB;
         CT X[2];                                      // transforms of the current pair's two rows
         static_for<0, NBF>([&](auto ri) {
@@ -276,8 +283,13 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
                 constexpr int j = decltype(ji)::value, q = j / 3, b = j % 3;
                 constexpr int r1 = FX::order(2 * q), r2 = FX::order(2 * q + 1);
                 R v1 = (R)0, v2 = (R)0;
-                if constexpr (r1 >= 0) v1 = FX::template row<r1>(vb[b]);
-                if constexpr (r2 >= 0) v2 = FX::template row<r2>(vb[b]);
+                if constexpr (LONE && q == QL) { // b = 0: phases 0 and 1; b = 1: phase 2
+                    v1 = FX::template row<r1>(vb[2 * b]);
+                    if constexpr (b == 0) v2 = FX::template row<r1>(vb[1]);
+                } else {
+                    if constexpr (r1 >= 0) v1 = FX::template row<r1>(vb[b]);
+                    if constexpr (r2 >= 0) v2 = FX::template row<r2>(vb[b]);
+                }
                 buf[(j - J0) * LPZ + lslot<LS>(r)] = CT{v1, v2};
             });
             __syncthreads();
@@ -288,15 +300,25 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
                 const CT zm = buf[(j - J0) * LPZ + lslot<LS>(rm)];
                 const CT F1 = CT{(R)0.5 * (zn.x + zm.x), (R)0.5 * (zn.y - zm.y)};
                 const CT F2 = CT{(R)0.5 * (zn.y + zm.y), (R)0.5 * (zm.x - zn.x)};
-                if constexpr (b == 0) {
-                    X[0] = F1;
-                    X[1] = F2;
+                if constexpr (LONE && q == QL) {
+                    if constexpr (b == 0) { // phase 0 (F1) and phase 1 (F2) of the one row
+                        X[0] = cadd(F1, cmul(F2, cconj(twq2<R>(twQl, NN, r))));
+                        X[1] = CT{(R)0, (R)0};
+                    } else { // phase 2
+                        X[0] = cadd(X[0], cmul(F1, cconj(twq2<R>(twQl, NN, 2 * r))));
+                        assemble(std::integral_constant<int, 2 * q>{}, X, nlr, nls);
+                    }
                 } else {
-                    const CT w = cconj(twq2<R>(twQl, NN, r * b));
-                    X[0] = cadd(X[0], cmul(F1, w));
-                    X[1] = cadd(X[1], cmul(F2, w));
+                    if constexpr (b == 0) {
+                        X[0] = F1;
+                        X[1] = F2;
+                    } else {
+                        const CT w = cconj(twq2<R>(twQl, NN, r * b));
+                        X[0] = cadd(X[0], cmul(F1, w));
+                        X[1] = cadd(X[1], cmul(F2, w));
+                    }
+                    if constexpr (b == 2) assemble(std::integral_constant<int, 2 * q>{}, X, nlr, nls);
                 }
-                if constexpr (b == 2) assemble(std::integral_constant<int, 2 * q>{}, X, nlr, nls);
             });
         });
         return;
