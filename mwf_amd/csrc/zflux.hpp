@@ -46,13 +46,23 @@ template <typename R, int LOGNN, bool PP> constexpr int zc_ls() { return zc_wave
 // sequences in the order (pair q, phase b) -> 3 q + b, so that every lane
 // group transforms; a pair whose three phases straddle two batches keeps its
 // partial sums in registers.  Pair-granular batches of ZP pairs (3 ZP = 6
-// sequences) leave two of the eight groups idle.  FP32 only: FP64's
-// transforms are bound by LDS bandwidth (16-byte values), and eight at once
-// made FP64 k_zc 2 % slower at M4 1024^2 despite one batch in nine fewer
-// (FP32 1 % faster).
+// sequences) leave two of the eight groups idle.  FP64 keeps batches of 6
+// (its transforms are bound by LDS bandwidth, 16-byte values, and eight at
+// once made FP64 k_zc 2 % slower at M4 1024^2 despite one batch in nine
+// fewer; FP32 1 % faster), except that a tail of up to 2 sequences joins the
+// last batch.
 template <typename R, int LOGNN, bool PP> constexpr bool zc_sg()
 {
-    return sizeof(R) == 4 && PP && zc_wave<R, LOGNN, PP>() && zc_threads<LOGNN>() == (1 << LOGNN);
+    return PP && zc_wave<R, LOGNN, PP>() && zc_threads<LOGNN>() == (1 << LOGNN);
+}
+// forward sequences per batch (the last batch may take up to 8)
+template <typename R> constexpr int zc_spb() { return sizeof(R) == 4 ? 8 : 6; }
+// forward batches of NSF sequences, SPB a batch, a tail of at most 8 - SPB
+// sequences joining the last full batch
+template <int NSF, int SPB> constexpr int zc_nbf()
+{
+    const int nb = (NSF + SPB - 1) / SPB;
+    return nb > 1 && NSF - (nb - 1) * SPB <= 8 - SPB ? nb - 1 : nb;
 }
 // sequences per buffer
 template <typename R, int LOGNN, int ZP, bool PP> constexpr int zc_bseq()
@@ -255,7 +265,8 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         });
     };
     if constexpr (SG) {
-        constexpr int SPB = T / group_lanes<NN>();   // sequences per forward batch
+        constexpr int SPB = zc_spb<R>();             // sequences per forward batch
+        static_assert(T / group_lanes<NN>() >= 8, "a batch of up to 8 sequences");
         // forward sequences: pair q, phase b at 3 q + b.  With an odd number
         // of rows the last pair holds one row, whose three phases take two
         // sequences (phases 0 and 1 as the real and imaginary parts of the
@@ -264,11 +275,11 @@ __global__ __launch_bounds__(zc_threads<LOGNN>(), WAVES) void k_zc(Batch bt, con
         constexpr bool LONE = NR % 2 == 1;
         constexpr int QL = NPO - 1;                  // the last pair
         constexpr int NSF = 3 * NPO - (LONE ? 1 : 0);
-        constexpr int NBF = (NSF + SPB - 1) / SPB;
+        constexpr int NBF = zc_nbf<NSF, SPB>();
         CT X[2];                                      // transforms of the current pair's two rows
         static_for<0, NBF>([&](auto ri) {
             constexpr int RD = decltype(ri)::value;
-            constexpr int J0 = RD * SPB, J1 = J0 + SPB < NSF ? J0 + SPB : NSF;
+            constexpr int J0 = RD * SPB, J1 = RD == NBF - 1 ? NSF : J0 + SPB;
             const int tid = opaque_tid();
             const int r = tid & (NN - 1); // one x-row per block
             const int rm = (NN - r) & (NN - 1);
